@@ -1,0 +1,149 @@
+/*
+ * labeling_amd.h -- C ABI of the MI355X-native connected-component labeler.
+ *
+ * Drop-in boundary for szavalishin/Labeling's algorithm-selection and
+ * label-call interface, restricted to the two GPU methods of that project:
+ *   "bleq"  block equivalence (2D on 2x2 blocks, 8-connectivity;
+ *           3D on 2x2x2 blocks, 26-connectivity)
+ *   "lbeq"  label equivalence after Kalentev (2D 4/8-connectivity, 3D 26)
+ * A binary image (or a slice stack) goes in, a uint32 label image comes out.
+ *
+ * Conventions
+ *  - Input: uint8, any nonzero byte is foreground.  2D images are row-major
+ *    [y][x] with a byte pitch; 3D volumes are dense [d][h][w] with w fastest.
+ *    The reference's 3D cv::Mat {rows, cols, slices} (slice fastest,
+ *    main.cpp:191,203) is passed as d = rows, h = cols, w = slices.
+ *  - Output: dense uint32, same shape as the input (3D is cropped, unlike the
+ *    reference, LabelingTools.cpp:217-220), background 0, allocated by the
+ *    caller.  Label values follow lbl_label_mode.
+ *  - Errors: every entry point returns lbl_status and never throws; the
+ *    message of the last failure on a context is lbl_last_error().  The
+ *    reference throws std::exception instead (LabelingTools.hpp:20-27).
+ *  - Threading: a context serialises its calls (the reference's labeler
+ *    objects are not re-entrant, LabelingAlgs.hpp:247-252).  Use one context
+ *    per host thread for concurrency.
+ *  - Streams: `stream` arguments are hipStream_t values passed as void*;
+ *    NULL selects the context's own stream.
+ */
+#ifndef LABELING_AMD_H_
+#define LABELING_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbl_ctx lbl_ctx;
+
+/* Registry names of ALG_LIST (main.cpp:117-126). */
+typedef enum { LBL_BLEQ = 0, LBL_LBEQ = 1 } lbl_alg;
+
+typedef enum {
+    LBL_OK = 0,
+    LBL_EINVAL = 1,          /* bad argument (null pointer, zero size, bad pitch) */
+    LBL_EUNSUPPORTED = 2,    /* e.g. bleq with 4-connectivity (LabelingAlgs.cpp:338,1070),
+                                3D with a connectivity other than 26 (LabelingTools.cpp:185),
+                                images beyond the index limits */
+    LBL_ENOMEM = 3,          /* device allocation failed, or a request exceeds a budget */
+    LBL_EDEVICE = 4,         /* a HIP runtime call failed */
+    LBL_EITER = 5,           /* an iteration cap was reached (debug knob) */
+    LBL_EBUSY = 6            /* resource temporarily unavailable (serving admission) */
+} lbl_status;
+
+/* Values written to foreground pixels.  All modes are deterministic. */
+typedef enum {
+    /* 1 + raster index (in the unpadded input) of the component's root:
+     * bleq: root = min 2x2 (2x2x2) block index, block grid ceil(h/2) x ceil(w/2);
+     * lbeq: root = min pixel (voxel) index.  Fastest mode; default. */
+    LBL_LABELS_ROOT = 0,
+    /* 2D bleq only: the reference OpenCL output values, 1 + min block index in
+     * the block grid of the image zero-padded to ((n>>5)<<5)+32
+     * (LabelingAlgs.cl:202,317; LabelingTools.cpp:103). */
+    LBL_LABELS_REFERENCE = 1,
+    /* 1..N in raster order of each component's first pixel (the canonical form
+     * used for parity).  Costs one extra pass. */
+    LBL_LABELS_CANONICAL = 2
+} lbl_label_mode;
+
+/* Connectivity argument: 0 = default (bleq 2D: 8, lbeq 2D: 4, 3D: 26),
+ * otherwise 4, 8 (2D) or 26 (3D).  Mirrors TCoherence (LabelingTools.hpp:45-50). */
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces the reference's labeler construction + IOCLLabeling::Init
+ * (LabelingTools.cpp:81-94): binds a device, creates a stream and events.
+ * Kernels are compiled offline for gfx950 (no run-time program build). */
+lbl_status lbl_create(lbl_ctx **out, int device_ordinal);
+
+/* Replaces ~IOCLLabeling / TerminateOCL (LabelingTools.cpp:136-154). */
+void lbl_destroy(lbl_ctx *ctx);
+
+/* Message of the last failed call on ctx (replaces the exception text of
+ * THROW_IF / THROW_IF_OCL, LabelingTools.hpp:20-27).  ctx may be NULL (then the
+ * message of the last failed lbl_create on this thread). */
+const char *lbl_last_error(const lbl_ctx *ctx);
+
+const char *lbl_status_string(lbl_status s);
+
+lbl_status lbl_set_label_mode(lbl_ctx *ctx, lbl_label_mode mode);
+
+/* Debug knob: cap on the number of tile-local label-equivalence sweeps (lbeq)
+ * per tile; exceeding it returns LBL_EITER.  0 = no cap (default). */
+lbl_status lbl_set_iteration_cap(lbl_ctx *ctx, uint32_t cap);
+
+/* Registry lookup by the reference's algorithm names, mirroring
+ * SetLabelingAlg (main.cpp:373-395): "bleq" or "lbeq".  Returns LBL_EUNSUPPORTED
+ * for names of algorithms this library does not provide ("bin", "runeq",
+ * "he-run", "gr-block", "ocv") and LBL_EINVAL for unknown names. */
+lbl_status lbl_alg_from_name(const char *name, lbl_alg *out);
+
+/* ---- single image, host buffers ------------------------------------------ */
+
+/* 2D labeling of a host image.  Replaces IOCLLabeling::Label
+ * (LabelingTools.hpp:110, LabelingTools.cpp:98-132) running
+ * TOCLLabelEquivalenceX2::DoOCLLabel (LabelingAlgs.cpp:1067-1084) for bleq or
+ * TOCLLabelDistribution::DoOCLLabel (LabelingAlgs.cpp:905-942) for lbeq.
+ * labels: h*w dense, caller-allocated.  n_components (nullable): number of
+ * components.  elapsed_us (nullable): device time of the labeling kernels
+ * only, excluding transfers and allocation -- the reference's timed region
+ * (LabelingTools.cpp:118-125). */
+lbl_status lbl_label2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *px, uint32_t h, uint32_t w,
+                       size_t pitch_bytes, int conn, uint32_t *labels, uint32_t *n_components,
+                       float *elapsed_us);
+
+/* 3D labeling of a host volume.  Replaces IOCLLabeling3D::Label
+ * (LabelingTools.hpp:154, LabelingTools.cpp:180-221) running
+ * TOCLBlockEquivalence3D::DoOCLLabel3D (LabelingAlgs.cpp:1520-1536) for bleq or
+ * TOCLLabelEquivalence3D::DoOCLLabel3D (LabelingAlgs.cpp:1362-1396) for lbeq.
+ * labels: d*h*w dense (cropped). */
+lbl_status lbl_label3d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *vox, uint32_t d, uint32_t h,
+                       uint32_t w, int conn, uint32_t *labels, uint32_t *n_components,
+                       float *elapsed_us);
+
+/* ---- single image, device buffers ----------------------------------------- */
+
+/* Same as lbl_label2d on device-resident buffers, enqueued on `stream` without
+ * any host synchronisation (the reference's DoOCLLabel body with its blocking
+ * per-iteration flag transfers removed, LabelingAlgs.cpp:1028-1038).
+ * d_ncomp (nullable, device): receives the component count. */
+lbl_status lbl_label2d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
+                           size_t pitch_bytes, int conn, uint32_t *d_labels, uint32_t *d_ncomp,
+                           void *stream);
+
+lbl_status lbl_label3d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_vox, uint32_t d, uint32_t h,
+                           uint32_t w, int conn, uint32_t *d_labels, uint32_t *d_ncomp, void *stream);
+
+/* Device time in microseconds of each kernel of the most recent *_dev call on
+ * ctx when profiling is enabled (lbl_set_profiling).  Writes up to `cap`
+ * entries to us[] and the kernel names to names[] (static strings), returns
+ * the number of kernels.  Blocks until that call's work is done. */
+lbl_status lbl_set_profiling(lbl_ctx *ctx, int enable);
+int lbl_last_kernel_times(lbl_ctx *ctx, float *us, const char **names, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LABELING_AMD_H_ */

@@ -1,0 +1,415 @@
+// capi.cpp -- extern "C" entry points declared in include/labeling_amd.h.
+//
+// Mirrors the reference's label-call contract (IOCLLabeling::Label,
+// LabelingTools.cpp:98-132; IOCLLabeling3D::Label, :180-221) without its
+// padding, per-call allocation and blocking per-iteration transfers: device
+// buffers are cached on the context, the labeling kernels are enqueued
+// back-to-back on one stream, and errors are returned as status codes.
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "lbl_internal.hpp"
+
+using namespace lbl;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+lbl_status fail(lbl_ctx *ctx, lbl_status st, const std::string &msg)
+{
+    if (ctx) ctx->last_error = msg;
+    else g_create_error = msg;
+    return st;
+}
+
+lbl_status hip_fail(lbl_ctx *ctx, const char *what, hipError_t e)
+{
+    return fail(ctx, e == hipErrorOutOfMemory ? LBL_ENOMEM : LBL_EDEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define LBL_HIP(ctx, expr)                                  \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(ctx, #expr, e_); \
+    } while (0)
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Grow a cached device buffer; waits for the device before freeing the old one.
+lbl_status ensure(lbl_ctx *ctx, void **buf, size_t *cap, size_t need)
+{
+    if (*cap >= need && *buf) return LBL_OK;
+    if (*buf) {
+        LBL_HIP(ctx, hipDeviceSynchronize());
+        LBL_HIP(ctx, hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
+    }
+    need = align256(need < 256 ? 256 : need);
+    hipError_t e = hipMalloc(buf, need);
+    if (e != hipSuccess) {
+        *buf = nullptr;
+        (void)hipGetLastError();
+        return fail(ctx, LBL_ENOMEM, "device allocation of " + std::to_string(need) + " bytes failed: " +
+                                         hipGetErrorString(e));
+    }
+    *cap = need;
+    return LBL_OK;
+}
+
+// Connectivity defaults and support, as in the reference:
+// bleq 2D: 8 (COH_4 rejected, LabelingAlgs.cpp:338,1070); lbeq 2D: default 4
+// (LabelingAlgs.cpp:213), 8 on request; 3D: default only = 26 (LabelingTools.cpp:185).
+lbl_status normalize_conn2d(lbl_ctx *ctx, lbl_alg alg, int conn, int *out)
+{
+    if (alg == LBL_BLEQ) {
+        if (conn == 0 || conn == 8) { *out = 8; return LBL_OK; }
+        if (conn == 4) return fail(ctx, LBL_EUNSUPPORTED, "bleq: method does not support 4-connectivity");
+    } else if (alg == LBL_LBEQ) {
+        if (conn == 0 || conn == 4) { *out = 4; return LBL_OK; }
+        if (conn == 8) { *out = 8; return LBL_OK; }
+    } else {
+        return fail(ctx, LBL_EINVAL, "unknown algorithm id " + std::to_string((int)alg));
+    }
+    return fail(ctx, LBL_EINVAL, "connectivity must be 0, 4 or 8 for 2D, got " + std::to_string(conn));
+}
+
+lbl_status normalize_conn3d(lbl_ctx *ctx, lbl_alg alg, int conn)
+{
+    if (alg != LBL_BLEQ && alg != LBL_LBEQ) return fail(ctx, LBL_EINVAL, "unknown algorithm id " + std::to_string((int)alg));
+    if (conn == 0 || conn == 26) return LBL_OK;
+    return fail(ctx, LBL_EUNSUPPORTED, "3D labeling supports only the default 26-connectivity");
+}
+
+lbl_status check_mode(lbl_ctx *ctx, lbl_alg alg, bool is3d)
+{
+    const lbl_label_mode m = ctx->opt.mode;
+    if (m == LBL_LABELS_REFERENCE && (alg != LBL_BLEQ || is3d))
+        return fail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_REFERENCE is defined for 2D bleq only");
+    if (m == LBL_LABELS_CANONICAL)
+        return fail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_CANONICAL is not available in this build");
+    return LBL_OK;
+}
+
+lbl_status check_size2d(lbl_ctx *ctx, lbl_alg alg, uint32_t h, uint32_t w, size_t pitch)
+{
+    if (h == 0 || w == 0) return fail(ctx, LBL_EINVAL, "input image is empty");
+    if (pitch < w) return fail(ctx, LBL_EINVAL, "pitch is smaller than the width");
+    if ((uint64_t)h * w > kMaxElems) return fail(ctx, LBL_EUNSUPPORTED, "image has more than 2^32-2 pixels");
+    if (alg == LBL_BLEQ && ((h + 1ull) / 2) * ((w + 1ull) / 2) > kMaxBlocks2D)
+        return fail(ctx, LBL_EUNSUPPORTED, "bleq 2D supports at most 2^28 2x2 blocks");
+    return LBL_OK;
+}
+
+lbl_status check_size3d(lbl_ctx *ctx, uint32_t d, uint32_t h, uint32_t w)
+{
+    if (d == 0 || h == 0 || w == 0) return fail(ctx, LBL_EINVAL, "input volume is empty");
+    if ((uint64_t)d * h * w > kMaxElems) return fail(ctx, LBL_EUNSUPPORTED, "volume has more than 2^32-2 voxels");
+    return LBL_OK;
+}
+
+size_t workspace2d(lbl_alg alg, uint32_t h, uint32_t w)
+{
+    return alg == LBL_BLEQ ? bleq2d_workspace(h, w) : lbeq2d_workspace(h, w);
+}
+
+size_t workspace3d(lbl_alg alg, uint32_t d, uint32_t h, uint32_t w)
+{
+    return alg == LBL_BLEQ ? bleq3d_workspace(d, h, w) : lbeq3d_workspace(d, h, w);
+}
+
+hipStream_t pick_stream(lbl_ctx *ctx, void *stream)
+{
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+}  // namespace
+
+// ---- KernelTimer ---------------------------------------------------------------
+
+void lbl::KernelTimer::begin(hipStream_t s)
+{
+    used = 0;
+    names.clear();
+    mark(s, nullptr);
+}
+
+void lbl::KernelTimer::mark(hipStream_t s, const char *name)
+{
+    if (used == events.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        events.push_back(e);
+    }
+    if (hipEventRecord(events[used], s) != hipSuccess) return;
+    ++used;
+    if (name) names.push_back(name);
+}
+
+// ---- context -----------------------------------------------------------------------
+
+extern "C" lbl_status lbl_create(lbl_ctx **out, int device_ordinal)
+{
+    if (!out) return fail(nullptr, LBL_EINVAL, "lbl_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(nullptr, LBL_EDEVICE, std::string("no HIP device available: ") + hipGetErrorString(e));
+    if (device_ordinal < 0 || device_ordinal >= n)
+        return fail(nullptr, LBL_EINVAL, "device ordinal out of range");
+    lbl_ctx *ctx = new (std::nothrow) lbl_ctx;
+    if (!ctx) return fail(nullptr, LBL_ENOMEM, "host allocation failed");
+    ctx->device = device_ordinal;
+    DeviceGuard g(device_ordinal);
+    lbl_status st = LBL_OK;
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void **>(&ctx->d_misc), 256)) != hipSuccess ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_misc), 256, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMemsetAsync(ctx->d_misc, 0, 256, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
+        st = fail(nullptr, LBL_EDEVICE, std::string("context setup failed: ") + hipGetErrorString(e));
+        lbl_destroy(ctx);
+        return st;
+    }
+    ctx->opt.status = ctx->d_misc + 1;
+    *out = ctx;
+    return LBL_OK;
+}
+
+extern "C" void lbl_destroy(lbl_ctx *ctx)
+{
+    if (!ctx) return;
+    {
+        DeviceGuard g(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->ws) (void)hipFree(ctx->ws);
+        if (ctx->d_in) (void)hipFree(ctx->d_in);
+        if (ctx->d_lab) (void)hipFree(ctx->d_lab);
+        if (ctx->d_misc) (void)hipFree(ctx->d_misc);
+        if (ctx->h_misc) (void)hipHostFree(ctx->h_misc);
+        if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+        if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+        for (hipEvent_t e : ctx->timer.events) (void)hipEventDestroy(e);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+extern "C" const char *lbl_last_error(const lbl_ctx *ctx)
+{
+    return ctx ? ctx->last_error.c_str() : g_create_error.c_str();
+}
+
+extern "C" const char *lbl_status_string(lbl_status s)
+{
+    switch (s) {
+    case LBL_OK: return "LBL_OK";
+    case LBL_EINVAL: return "LBL_EINVAL";
+    case LBL_EUNSUPPORTED: return "LBL_EUNSUPPORTED";
+    case LBL_ENOMEM: return "LBL_ENOMEM";
+    case LBL_EDEVICE: return "LBL_EDEVICE";
+    case LBL_EITER: return "LBL_EITER";
+    case LBL_EBUSY: return "LBL_EBUSY";
+    }
+    return "unknown status";
+}
+
+extern "C" lbl_status lbl_set_label_mode(lbl_ctx *ctx, lbl_label_mode mode)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (mode != LBL_LABELS_ROOT && mode != LBL_LABELS_REFERENCE && mode != LBL_LABELS_CANONICAL)
+        return fail(ctx, LBL_EINVAL, "unknown label mode");
+    ctx->opt.mode = mode;
+    return LBL_OK;
+}
+
+extern "C" lbl_status lbl_set_iteration_cap(lbl_ctx *ctx, uint32_t cap)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->opt.iter_cap = cap;
+    return LBL_OK;
+}
+
+extern "C" lbl_status lbl_alg_from_name(const char *name, lbl_alg *out)
+{
+    if (!name || !out) return LBL_EINVAL;
+    if (!std::strcmp(name, "bleq")) { *out = LBL_BLEQ; return LBL_OK; }
+    if (!std::strcmp(name, "lbeq")) { *out = LBL_LBEQ; return LBL_OK; }
+    static const char *other[] = {"bin", "runeq", "he-run", "gr-block", "ocv"};   // main.cpp:103-130
+    for (const char *o : other)
+        if (!std::strcmp(name, o)) return LBL_EUNSUPPORTED;
+    return LBL_EINVAL;
+}
+
+extern "C" lbl_status lbl_set_profiling(lbl_ctx *ctx, int enable)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->profiling = enable != 0;
+    return LBL_OK;
+}
+
+extern "C" int lbl_last_kernel_times(lbl_ctx *ctx, float *us, const char **names, int cap)
+{
+    if (!ctx) return -1;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    KernelTimer &kt = ctx->timer;
+    if (kt.used < 2) return 0;
+    if (hipEventSynchronize(kt.events[kt.used - 1]) != hipSuccess) return -1;
+    int n = (int)kt.used - 1;
+    for (int i = 0; i < n && i < cap; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, kt.events[i], kt.events[i + 1]) != hipSuccess) ms = -1.f;
+        if (us) us[i] = ms * 1000.f;
+        if (names) names[i] = i < (int)kt.names.size() ? kt.names[i] : "";
+    }
+    return n;
+}
+
+// ---- device-buffer entry points ----------------------------------------------
+
+static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s)
+{
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w));
+    if (st != LBL_OK) return st;
+    KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
+    hipError_t e = alg == LBL_BLEQ ? bleq2d_run(im, ctx->opt, ctx->ws, s, kt)
+                                   : lbeq2d_run(im, conn, ctx->opt, ctx->ws, s, kt);
+    if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq2d launch" : "lbeq2d launch", e);
+    return LBL_OK;
+}
+
+static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_t s)
+{
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace3d(alg, im.d, im.h, im.w));
+    if (st != LBL_OK) return st;
+    KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
+    hipError_t e = alg == LBL_BLEQ ? bleq3d_run(im, ctx->opt, ctx->ws, s, kt) : lbeq3d_run(im, ctx->opt, ctx->ws, s, kt);
+    if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq3d launch" : "lbeq3d launch", e);
+    return LBL_OK;
+}
+
+extern "C" lbl_status lbl_label2d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
+                                      size_t pitch_bytes, int conn, uint32_t *d_labels, uint32_t *d_ncomp,
+                                      void *stream)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!d_px || !d_labels) return fail(ctx, LBL_EINVAL, "lbl_label2d_dev: null buffer");
+    int c = 0;
+    lbl_status st;
+    if ((st = normalize_conn2d(ctx, alg, conn, &c)) != LBL_OK) return st;
+    if ((st = check_size2d(ctx, alg, h, w, pitch_bytes)) != LBL_OK) return st;
+    if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
+    DeviceGuard g(ctx->device);
+    Image2D im{d_px, pitch_bytes, h, w, d_labels, d_ncomp};
+    return run2d(ctx, alg, c, im, pick_stream(ctx, stream));
+}
+
+extern "C" lbl_status lbl_label3d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_vox, uint32_t d, uint32_t h,
+                                      uint32_t w, int conn, uint32_t *d_labels, uint32_t *d_ncomp, void *stream)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!d_vox || !d_labels) return fail(ctx, LBL_EINVAL, "lbl_label3d_dev: null buffer");
+    lbl_status st;
+    if ((st = normalize_conn3d(ctx, alg, conn)) != LBL_OK) return st;
+    if ((st = check_size3d(ctx, d, h, w)) != LBL_OK) return st;
+    if ((st = check_mode(ctx, alg, true)) != LBL_OK) return st;
+    DeviceGuard g(ctx->device);
+    Image3D im{d_vox, d, h, w, d_labels, d_ncomp};
+    return run3d(ctx, alg, im, pick_stream(ctx, stream));
+}
+
+// ---- host-buffer entry points --------------------------------------------------
+
+static lbl_status finish_host(lbl_ctx *ctx, uint32_t *labels, size_t n, uint32_t *n_components, float *elapsed_us)
+{
+    hipStream_t s = ctx->stream;
+    LBL_HIP(ctx, hipEventRecord(ctx->ev1, s));
+    LBL_HIP(ctx, hipMemcpyAsync(labels, ctx->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LBL_HIP(ctx, hipMemcpyAsync(ctx->h_misc, ctx->d_misc, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LBL_HIP(ctx, hipStreamSynchronize(s));
+    if (elapsed_us) {
+        float ms = 0.f;
+        LBL_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *elapsed_us = ms * 1000.f;
+    }
+    if (n_components) *n_components = ctx->h_misc[0];
+    if (ctx->h_misc[1] & 1u) {
+        LBL_HIP(ctx, hipMemsetAsync(ctx->d_misc + 1, 0, sizeof(uint32_t), s));
+        LBL_HIP(ctx, hipStreamSynchronize(s));
+        return fail(ctx, LBL_EITER, "iteration cap reached");
+    }
+    return LBL_OK;
+}
+
+extern "C" lbl_status lbl_label2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *px, uint32_t h, uint32_t w,
+                                  size_t pitch_bytes, int conn, uint32_t *labels, uint32_t *n_components,
+                                  float *elapsed_us)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!px || !labels) return fail(ctx, LBL_EINVAL, "lbl_label2d: null buffer");
+    int c = 0;
+    lbl_status st;
+    if ((st = normalize_conn2d(ctx, alg, conn, &c)) != LBL_OK) return st;
+    if ((st = check_size2d(ctx, alg, h, w, pitch_bytes)) != LBL_OK) return st;
+    if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
+    DeviceGuard g(ctx->device);
+    const size_t n = (size_t)h * w;
+    const size_t dpitch = (w + 15u) & ~size_t(15);   // 16-byte rows for vector loads
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_in), &ctx->in_bytes, dpitch * h)) != LBL_OK) return st;
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t))) != LBL_OK)
+        return st;
+    hipStream_t s = ctx->stream;
+    LBL_HIP(ctx, hipMemcpy2DAsync(ctx->d_in, dpitch, px, pitch_bytes, w, h, hipMemcpyHostToDevice, s));
+    LBL_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    Image2D im{ctx->d_in, dpitch, h, w, ctx->d_lab, ctx->d_misc};
+    if ((st = run2d(ctx, alg, c, im, s)) != LBL_OK) return st;
+    return finish_host(ctx, labels, n, n_components, elapsed_us);
+}
+
+extern "C" lbl_status lbl_label3d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *vox, uint32_t d, uint32_t h,
+                                  uint32_t w, int conn, uint32_t *labels, uint32_t *n_components, float *elapsed_us)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!vox || !labels) return fail(ctx, LBL_EINVAL, "lbl_label3d: null buffer");
+    lbl_status st;
+    if ((st = normalize_conn3d(ctx, alg, conn)) != LBL_OK) return st;
+    if ((st = check_size3d(ctx, d, h, w)) != LBL_OK) return st;
+    if ((st = check_mode(ctx, alg, true)) != LBL_OK) return st;
+    DeviceGuard g(ctx->device);
+    const size_t n = (size_t)d * h * w;
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_in), &ctx->in_bytes, n)) != LBL_OK) return st;
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t))) != LBL_OK)
+        return st;
+    hipStream_t s = ctx->stream;
+    LBL_HIP(ctx, hipMemcpyAsync(ctx->d_in, vox, n, hipMemcpyHostToDevice, s));
+    LBL_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    Image3D im{ctx->d_in, d, h, w, ctx->d_lab, ctx->d_misc};
+    if ((st = run3d(ctx, alg, im, s)) != LBL_OK) return st;
+    return finish_host(ctx, labels, n, n_components, elapsed_us);
+}

@@ -1,0 +1,90 @@
+// lbl_internal.hpp -- context, error plumbing and launch descriptors shared by
+// the C ABI and the HIP kernels.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/labeling_amd.h"
+
+namespace lbl {
+
+// Launch description of one 2D image, in the units the kernels use.
+struct Image2D {
+    const uint8_t *px;     // device
+    size_t pitch;          // bytes
+    uint32_t h, w;
+    uint32_t *labels;      // device, dense h*w
+    uint32_t *count;       // device, component count (written)
+};
+
+struct Image3D {
+    const uint8_t *vox;    // device, dense d*h*w
+    uint32_t d, h, w;
+    uint32_t *labels;      // device, dense
+    uint32_t *count;       // device
+};
+
+// Per-call settings shared by every algorithm.
+struct Options {
+    lbl_label_mode mode = LBL_LABELS_ROOT;
+    uint32_t iter_cap = 0;       // lbeq tile sweeps cap, 0 = none
+    uint32_t *status = nullptr;  // device word; kernels OR in error bits (bit0 = iteration cap hit)
+};
+
+// A named kernel launch recorded for optional per-kernel timing.
+struct KernelTimer {
+    bool enabled = false;
+    std::vector<hipEvent_t> events;      // events[i] .. events[i+1] brackets kernel i
+    std::vector<const char *> names;
+    size_t used = 0;
+    void begin(hipStream_t s);
+    void mark(hipStream_t s, const char *name);
+};
+
+// Workspace sizes (bytes) needed by each path; 256-byte aligned sub-buffers.
+size_t bleq2d_workspace(uint32_t h, uint32_t w);
+size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w);
+size_t lbeq2d_workspace(uint32_t h, uint32_t w);
+size_t lbeq3d_workspace(uint32_t d, uint32_t h, uint32_t w);
+
+// Enqueue the full labeling pipeline of one image on `s`.  `ws` must hold the
+// workspace size above.  Return hipSuccess or the first launch error.
+hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+
+// Limits checked on the host before any launch.
+constexpr uint64_t kMaxBlocks2D = (1ull << 28);   // 2D block word packs parent << 4 | mask
+constexpr uint64_t kMaxElems = (1ull << 32) - 2;  // labels are uint32 with 0 = background
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace lbl
+
+// The context object behind the opaque lbl_ctx handle.
+struct lbl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    std::string last_error;
+    lbl::Options opt;
+    bool profiling = false;
+    lbl::KernelTimer timer;
+
+    // cached device buffers
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    uint8_t *d_in = nullptr;
+    size_t in_bytes = 0;
+    uint32_t *d_lab = nullptr;
+    size_t lab_bytes = 0;
+    uint32_t *d_misc = nullptr;   // [0] = component count, [1] = status bits
+    uint32_t *h_misc = nullptr;   // pinned host mirror of d_misc
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};

@@ -1,0 +1,216 @@
+// uf_device.hpp -- lock-free union-find primitives used by every labeling kernel.
+//
+// Invariant everywhere: parent(x) <= x, so the root of a set is its minimum
+// element and links only ever decrease.  Linking uses atomicMin on the larger
+// root: if the target stopped being a root concurrently, atomicMin keeps the
+// smaller of the two parents and the loop continues with the displaced parent,
+// so no equivalence is lost.  max(a, b) strictly decreases on every retry, so
+// the loops terminate.  Stale reads are harmless: values only decrease and any
+// value ever stored is an element of the same set.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lbl {
+
+// ---- loads that observe concurrent atomics --------------------------------
+
+__device__ __forceinline__ uint32_t lds_load(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void lds_store(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ uint32_t dev_load(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void dev_store(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t dev_min(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t lds_min(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ---- LDS forest: par[x] holds the parent index ------------------------------
+
+__device__ __forceinline__ uint32_t lds_find(uint32_t *par, uint32_t x)
+{
+    uint32_t p = lds_load(&par[x]);
+    while (p != x) {
+        x = p;
+        p = lds_load(&par[x]);
+    }
+    return x;
+}
+
+__device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b)
+{
+    for (;;) {
+        a = lds_find(par, a);
+        b = lds_find(par, b);
+        if (a == b) return;
+        if (a < b) { uint32_t t = a; a = b; b = t; }
+        uint32_t old = lds_min(&par[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+// ---- global forest with a packed low field ---------------------------------
+// word = parent << SHIFT | low, where `low` (the block's foreground mask) never
+// changes.  SHIFT = 0 with an all-zero low field is the plain parent array.
+
+template <int SHIFT>
+struct PackedForest {
+    static constexpr uint32_t kLowMask = (1u << SHIFT) - 1u;
+
+    // Root of x with path halving (x -> grandparent, by atomicMin so a link
+    // never grows).
+    __device__ static uint32_t find(uint32_t *w, uint32_t x)
+    {
+        uint32_t v = dev_load(&w[x]);
+        for (;;) {
+            uint32_t p = v >> SHIFT;
+            if (p == x) return x;
+            uint32_t vp = dev_load(&w[p]);
+            uint32_t gp = vp >> SHIFT;
+            if (gp == p) return p;
+            dev_min(&w[x], (gp << SHIFT) | (v & kLowMask));
+            x = gp;
+            v = dev_load(&w[x]);
+        }
+    }
+
+    // Root of x when nothing writes the forest any more (plain cached loads).
+    __device__ static uint32_t find_final(const uint32_t *__restrict__ w, uint32_t x)
+    {
+        uint32_t v = w[x];
+        while ((v >> SHIFT) != x) {
+            x = v >> SHIFT;
+            v = w[x];
+        }
+        return x;
+    }
+
+    __device__ static void unite(uint32_t *w, uint32_t a, uint32_t b)
+    {
+        for (;;) {
+            a = find(w, a);
+            b = find(w, b);
+            if (a == b) return;
+            if (a < b) { uint32_t t = a; a = b; b = t; }
+            uint32_t va = dev_load(&w[a]);
+            if ((va >> SHIFT) != a) continue;   // a was linked meanwhile: re-find
+            uint32_t old = dev_min(&w[a], (b << SHIFT) | (va & kLowMask));
+            if ((old >> SHIFT) == a) return;
+            a = old >> SHIFT;
+        }
+    }
+
+    // After all unions: point x's entry straight at its root.
+    __device__ static void compress(uint32_t *w, uint32_t x)
+    {
+        uint32_t v = dev_load(&w[x]);
+        if ((v >> SHIFT) == x) return;
+        uint32_t r = find(w, x);
+        dev_min(&w[x], (r << SHIFT) | (v & kLowMask));
+    }
+};
+
+// ---- label forest: label = index + 1, 0 = background -----------------------
+// Used by label equivalence, where the output label image itself is the
+// equivalence array (as in the reference, LabelingAlgs.cl:105-106,275-276).
+
+struct LabelForest {
+    __device__ static uint32_t find(uint32_t *L, uint32_t x)
+    {
+        uint32_t v = dev_load(&L[x]);
+        for (;;) {
+            uint32_t p = v - 1u;
+            if (p == x) return x;
+            uint32_t vp = dev_load(&L[p]);
+            uint32_t gp = vp - 1u;
+            if (gp == p) return p;
+            dev_min(&L[x], vp);
+            x = gp;
+            v = dev_load(&L[x]);
+        }
+    }
+
+    __device__ static void unite(uint32_t *L, uint32_t a, uint32_t b)
+    {
+        for (;;) {
+            a = find(L, a);
+            b = find(L, b);
+            if (a == b) return;
+            if (a < b) { uint32_t t = a; a = b; b = t; }
+            uint32_t old = dev_min(&L[a], b + 1u);
+            if (old == a + 1u) return;
+            a = old - 1u;
+        }
+    }
+
+    __device__ static void compress(uint32_t *L, uint32_t x)
+    {
+        uint32_t v = dev_load(&L[x]);
+        if (v - 1u == x) return;
+        uint32_t r = find(L, x);
+        dev_min(&L[x], r + 1u);
+    }
+};
+
+// ---- vector memory helpers --------------------------------------------------
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte streaming store (bypasses cache retention): for label output that is
+// written once and not read back by the kernels.
+__device__ __forceinline__ void store_nt4(uint32_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    u32x4 v = {a, b, c, d};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(p));
+}
+
+// ---- wave helpers ----------------------------------------------------------
+
+// Add `v` over the workgroup's waves into *dst with one atomic per wave.
+__device__ __forceinline__ void wave_add_atomic(uint32_t *dst, uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63u) == 0 && v) atomicAdd(dst, v);
+}
+
+// Nonzero-byte test: bit 8k of the result is set iff byte k of v is nonzero.
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t v)
+{
+    uint32_t t = ((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v;
+    return (t >> 7) & 0x01010101u;
+}
+
+// Two 4-pixel words of consecutive rows -> the 2x2 masks of the two blocks
+// they cover, as two nibbles (bit0 = top-left, bit1 = top-right,
+// bit2 = bottom-left, bit3 = bottom-right).
+__device__ __forceinline__ uint32_t quad_masks(uint32_t top, uint32_t bot)
+{
+    uint32_t t = nz_bytes(top), b = nz_bytes(bot);
+    return (t & 1u) | ((t >> 7) & 2u) | ((b & 1u) << 2) | ((b >> 5) & 8u) |
+           ((t >> 12) & 0x10u) | ((t >> 19) & 0x20u) | ((b >> 10) & 0x40u) | ((b >> 17) & 0x80u);
+}
+
+}  // namespace lbl

@@ -1,0 +1,175 @@
+"""Writes the golden fixtures used by the parity tests.
+
+Labels come from scipy.ndimage.label (a library routine independent of both
+the reference and this project), canonicalised with numpy: components renumbered
+1..N in memory order of their first pixel.  Inputs come from the project's
+counter-based generator (labeling_amd/libsynth.so), so the same bytes are
+produced on any machine; each input's SHA-256 is recorded to pin that.
+
+  small2d.npz / small3d.npz  small cases in full (inputs + canonical labels)
+  large.json                 full-size workloads: input and label hashes, counts
+
+Run from the repository root:  python tests/golden/make_goldens.py [--skip-large]
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+from scipy import ndimage
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import labeling_amd as L  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+S8 = np.ones((3, 3), dtype=bool)
+S4 = ndimage.generate_binary_structure(2, 1)
+S26 = np.ones((3, 3, 3), dtype=bool)
+
+
+def canonical(lab):
+    flat = lab.ravel()
+    nz = flat != 0
+    if not nz.any():
+        return np.zeros(lab.shape, np.uint32), 0
+    vals, first = np.unique(flat[nz], return_index=True)
+    order = np.argsort(first, kind="stable")
+    remap = np.zeros(int(vals.max()) + 1, dtype=np.uint32)
+    remap[vals[order]] = np.arange(1, len(vals) + 1, dtype=np.uint32)
+    out = np.zeros(flat.shape, np.uint32)
+    out[nz] = remap[flat[nz]]
+    return out.reshape(lab.shape), len(vals)
+
+
+def label(img, structure):
+    lab, n = ndimage.label(img != 0, structure=structure)
+    c, m = canonical(lab)
+    assert m == n
+    return c, n
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def serpentine(h, w):
+    """Full rows joined alternately at the right and left ends: one long snake,
+    the worst case for iterative label propagation."""
+    img = np.zeros((h, w), np.uint8)
+    for r in range(0, h, 2):
+        img[r, :] = 255
+        if r + 1 < h:
+            img[r + 1, w - 1 if (r // 2) % 2 == 0 else 0] = 255
+    return img
+
+
+def small_2d_cases():
+    cases = {}
+    for i, p in enumerate([0.1, 0.3, 0.5, 0.7, 0.9]):
+        cases[f"bern64_p{int(p * 10)}"] = L.bernoulli((64, 64), p, seed=100 + i)
+    cases["bern37x53"] = L.bernoulli((37, 53), 0.5, seed=7)
+    cases["bern33x31"] = L.bernoulli((33, 31), 0.45, seed=8)
+    cases["bern1x100"] = L.bernoulli((1, 100), 0.6, seed=9)
+    cases["bern100x1"] = L.bernoulli((100, 1), 0.6, seed=10)
+    cases["bern63x95"] = L.bernoulli((63, 95), 0.55, seed=11)
+    cases["blob96"] = L.blobs2d(96, 96, 4.0, 0.5, seed=12)
+    cases["blob70x130"] = L.blobs2d(70, 130, 3.0, 0.4, seed=13)
+    yy, xx = np.mgrid[0:16, 0:16]
+    cases["checker16"] = np.where((yy + xx) % 2 == 0, 255, 0).astype(np.uint8)
+    cases["zeros8"] = np.zeros((8, 8), np.uint8)
+    cases["ones9x7"] = np.full((9, 7), 255, np.uint8)
+    cases["one1x1"] = np.full((1, 1), 255, np.uint8)
+    cases["serpentine33"] = serpentine(33, 41)
+    kat1 = np.zeros((4, 4), np.uint8)
+    kat1[0, 3] = kat1[1, 0] = 255
+    cases["kat1_wrap"] = kat1
+    kat2 = np.zeros((4, 4), np.uint8)
+    kat2[1, 1] = kat2[2, 2] = 255
+    cases["kat2_corner"] = kat2
+    kat3 = np.zeros((3, 3), np.uint8)
+    kat3[0, 0] = kat3[0, 1] = kat3[1, 0] = 255
+    cases["kat3_origin"] = kat3
+    return cases
+
+
+def small_3d_cases():
+    cases = {}
+    cases["bern16_p1"] = L.bernoulli((16, 16, 16), 0.1, seed=200)
+    cases["bern16_p3"] = L.bernoulli((16, 16, 16), 0.3, seed=201)
+    cases["bern9x13x11"] = L.bernoulli((9, 13, 11), 0.3, seed=202)
+    cases["bern1x17x5"] = L.bernoulli((1, 17, 5), 0.5, seed=203)
+    cases["blob24"] = L.blobs3d(24, 24, 24, 3.0, 0.4, seed=204)
+    kat = np.zeros((4, 4, 4), np.uint8)
+    kat[1, 1, 1] = kat[2, 2, 2] = 255
+    cases["kat2_corner3d"] = kat
+    cases["ones3x5x7"] = np.full((3, 5, 7), 255, np.uint8)
+    return cases
+
+
+def write_small():
+    arrays = {}
+    for name, img in small_2d_cases().items():
+        arrays[f"{name}__img"] = img
+        arrays[f"{name}__l8"], _ = label(img, S8)
+        arrays[f"{name}__l4"], _ = label(img, S4)
+    np.savez_compressed(os.path.join(OUT, "small2d.npz"), **arrays)
+    arrays = {}
+    for name, vol in small_3d_cases().items():
+        arrays[f"{name}__img"] = vol
+        arrays[f"{name}__l26"], _ = label(vol, S26)
+    np.savez_compressed(os.path.join(OUT, "small3d.npz"), **arrays)
+
+
+def large_entries():
+    """(name, generator spec, structures) of the full-size workloads."""
+    out = []
+    for s in (0, 1, 2):
+        out.append((f"c1_bern1024_p5_s{s}", {"kind": "bernoulli", "shape": [1024, 1024], "p": 0.5, "seed": s},
+                    ["8", "4"]))
+    for p in (0.1, 0.3, 0.5, 0.7, 0.9):
+        out.append((f"c2_bern8192_p{int(p * 10)}", {"kind": "bernoulli", "shape": [8192, 8192], "p": p, "seed": 0},
+                    ["8"]))
+    out.append(("c2_blob8192", {"kind": "blobs2d", "shape": [8192, 8192], "sigma": 4.0, "q": 0.5, "seed": 0}, ["8"]))
+    for p in (0.1, 0.3):
+        out.append((f"c3_bern512_p{int(p * 10)}", {"kind": "bernoulli", "shape": [512, 512, 512], "p": p,
+                                                   "seed": 0}, ["26"]))
+    out.append(("c3_blob512", {"kind": "blobs3d", "shape": [512, 512, 512], "sigma": 3.0, "q": 0.4, "seed": 0},
+                ["26"]))
+    return out
+
+
+def generate(spec):
+    if spec["kind"] == "bernoulli":
+        return L.bernoulli(tuple(spec["shape"]), spec["p"], spec["seed"])
+    if spec["kind"] == "blobs2d":
+        return L.blobs2d(*spec["shape"], spec["sigma"], spec["q"], spec["seed"])
+    return L.blobs3d(*spec["shape"], spec["sigma"], spec["q"], spec["seed"])
+
+
+def write_large():
+    structs = {"8": S8, "4": S4, "26": S26}
+    res = {}
+    for name, spec, conns in large_entries():
+        img = generate(spec)
+        entry = {"spec": spec, "input_sha256": sha(img), "foreground": int(np.count_nonzero(img))}
+        for c in conns:
+            lab, n = label(img, structs[c])
+            entry[f"conn{c}"] = {"labels_sha256": sha(lab), "components": int(n)}
+            del lab
+        res[name] = entry
+        print(name, entry, flush=True)
+        del img
+    with open(os.path.join(OUT, "large.json"), "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-large", action="store_true")
+    a = ap.parse_args()
+    write_small()
+    if not a.skip_large:
+        write_large()

@@ -1,0 +1,166 @@
+"""GPU parity of 2D block equivalence (bleq) against the oracle, through the C ABI.
+
+Bar: bit-exact.  Canonical labels and component counts must equal the oracle's
+(and the scipy goldens); in LBL_LABELS_REFERENCE mode the raw label values
+must equal the reference OpenCL values restated by the oracle
+(1 + min padded-block index, LabelingAlgs.cl:202,317).
+"""
+import ctypes
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import labeling_amd as L
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _small2d():
+    z = np.load(os.path.join(GOLD, "small2d.npz"))
+    names = sorted({k.split("__")[0] for k in z.files})
+    return [(n, z[f"{n}__img"], z[f"{n}__l8"]) for n in names]
+
+
+@pytest.fixture
+def ref_mode(labeler):
+    labeler.set_mode(L.LABELS_REFERENCE)
+    yield labeler
+    labeler.set_mode(L.LABELS_ROOT)
+
+
+@pytest.mark.parametrize("case", _small2d(), ids=lambda c: c[0])
+def test_small_goldens(labeler, case):
+    name, img, l8 = case
+    print(f"[bleq2d golden {name} {img.shape}]", flush=True)
+    lab, n, _ = labeler.label2d(img, "bleq")
+    c, m = O.canonical(lab)
+    np.testing.assert_array_equal(c, l8)
+    assert n == m == int(l8.max())
+
+
+SHAPES = [(1, 1), (2, 2), (1, 300), (300, 1), (3, 5), (31, 257), (32, 256), (33, 255), (34, 258), (64, 512),
+          (65, 513), (257, 511), (100, 1000), (1000, 100), (517, 1029), (1024, 1024)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("p", [0.2, 0.5, 0.8])
+def test_reference_raw_values(ref_mode, shape, p):
+    img = L.bernoulli(shape, p, seed=hash((shape, p)) & 0xFFFF)
+    print(f"[bleq2d raw {shape} p={p}]", flush=True)
+    lab, n, _ = ref_mode.label2d(img, "bleq")
+    expect = O.bleq2d(img)            # padded reference restatement: raw values
+    np.testing.assert_array_equal(lab, expect)
+    assert n == O.canonical(expect)[1]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_root_labels_vs_bfs(labeler, shape):
+    img = L.bernoulli(shape, 0.55, seed=17)
+    print(f"[bleq2d root {shape}]", flush=True)
+    lab, n, _ = labeler.label2d(img, "bleq")
+    canon, m = O.bfs2d(img, 8)
+    c, k = O.canonical(lab)
+    np.testing.assert_array_equal(c, canon)
+    assert n == m == k
+    # root labels: 1 + min index in the unpadded block grid
+    bw = (shape[1] + 1) // 2
+    ys, xs = np.nonzero(img)
+    if len(ys):
+        blk = (ys // 2) * bw + xs // 2
+        comp = canon[ys, xs]
+        mins = np.full(m + 1, np.iinfo(np.int64).max)
+        np.minimum.at(mins, comp, blk)
+        np.testing.assert_array_equal(lab[ys, xs], mins[comp] + 1)
+
+
+def test_blobs_and_serpentine(labeler):
+    img = L.blobs2d(777, 1555, 4.0, 0.5, seed=3)
+    lab, n, _ = labeler.label2d(img, "bleq")
+    np.testing.assert_array_equal(O.canonical(lab)[0], O.bfs2d(img, 8)[0])
+    ser = np.zeros((301, 999), np.uint8)
+    for r in range(0, 301, 2):
+        ser[r, :] = 255
+        if r + 1 < 301:
+            ser[r + 1, 998 if (r // 2) % 2 == 0 else 0] = 255
+    lab, n, _ = labeler.label2d(ser, "bleq")
+    assert n == 1 and O.canonical(lab)[1] == 1
+
+
+def test_deterministic(labeler):
+    img = L.bernoulli((1500, 2500), 0.5, seed=5)
+    a = labeler.label2d(img, "bleq")[0]
+    for _ in range(3):
+        np.testing.assert_array_equal(labeler.label2d(img, "bleq")[0], a)
+
+
+def test_host_pitch(labeler):
+    """Rows with a pitch larger than the width (host entry point)."""
+    h, w, pitch = 123, 301, 333
+    buf = L.bernoulli((h, pitch), 0.5, seed=9)
+    out = np.empty((h, w), np.uint32)
+    n = ctypes.c_uint32()
+    st = labeler.lib.lbl_label2d(labeler.handle, L.BLEQ, buf.ctypes.data, h, w, pitch, 0, out.ctypes.data,
+                                 ctypes.byref(n), None)
+    assert st == L.OK
+    img = np.ascontiguousarray(buf[:, :w])
+    np.testing.assert_array_equal(O.canonical(out)[0], O.bfs2d(img, 8)[0])
+
+
+def test_device_entry_point_unaligned_pitch(labeler):
+    """Device buffers with a pitch that is not a multiple of 16 (scalar load path)."""
+    torch = pytest.importorskip("torch")
+    h, w, pitch = 257, 1001, 1003
+    host = L.bernoulli((h, pitch), 0.45, seed=21)
+    d_px = torch.from_numpy(host).cuda()
+    d_lab = torch.empty((h, w), dtype=torch.int32, device="cuda")
+    d_n = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    labeler.label2d_dev(d_px.data_ptr(), h, w, pitch, d_lab.data_ptr(), "bleq", 0, d_n.data_ptr(), stream)
+    torch.cuda.synchronize()
+    lab = d_lab.cpu().numpy().view(np.uint32)
+    canon, m = O.bfs2d(np.ascontiguousarray(host[:, :w]), 8)
+    np.testing.assert_array_equal(O.canonical(lab)[0], canon)
+    assert int(d_n.item()) == m
+
+
+def test_errors(labeler):
+    img = np.zeros((4, 4), np.uint8)
+    with pytest.raises(L.LabelingError) as e:
+        labeler.label2d(img, "bleq", conn=4)
+    assert e.value.status == L.EUNSUPPORTED
+    with pytest.raises(L.LabelingError) as e:
+        labeler.label2d(img, "bleq", conn=6)
+    assert e.value.status == L.EINVAL
+    with pytest.raises(L.LabelingError) as e:
+        labeler.label2d(np.zeros((0, 4), np.uint8), "bleq")
+    assert e.value.status == L.EINVAL
+
+
+def _large():
+    path = os.path.join(GOLD, "large.json")
+    return json.load(open(path)) if os.path.exists(path) else {}
+
+
+FULL = [k for k in sorted(_large()) if k.startswith("c2_") or k.startswith("c1_")]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", FULL)
+def test_full_size_vs_scipy_golden(labeler, name):
+    entry = _large()[name]
+    spec = entry["spec"]
+    print(f"[bleq2d full {name}]", flush=True)
+    if spec["kind"] == "bernoulli":
+        img = L.bernoulli(tuple(spec["shape"]), spec["p"], spec["seed"])
+    else:
+        img = L.blobs2d(*spec["shape"], spec["sigma"], spec["q"], spec["seed"])
+    assert hashlib.sha256(img.tobytes()).hexdigest() == entry["input_sha256"]
+    lab, n, _ = labeler.label2d(img, "bleq")
+    c, m = O.canonical(lab)
+    assert hashlib.sha256(c.tobytes()).hexdigest() == entry["conn8"]["labels_sha256"]
+    assert n == m == entry["conn8"]["components"]
