@@ -211,7 +211,8 @@ __global__ __launch_bounds__(256) void bleq2d_merge(uint32_t *P, uint32_t bw, ui
     border_block(j, bx0, by0, bx, by);
     if (bx >= bw || by >= bh) return;
     const uint32_t b = by * bw + bx;
-    const uint32_t m = P[b] & 15u;
+    const uint32_t vb = P[b];
+    const uint32_t m = vb & 15u;
     if (!m) return;
     // earlier neighbours: left, up-left, up, up-right
     const int ndx[4] = {-1, -1, 0, 1}, ndy[4] = {0, -1, -1, -1};
@@ -221,9 +222,10 @@ __global__ __launch_bounds__(256) void bleq2d_merge(uint32_t *P, uint32_t bw, ui
         if (nx >= bw || ny >= bh) continue;
         if (nx - bx0 < TBW && ny - by0 < TBH) continue;    // same tile: done in the tile kernel
         const uint32_t n = ny * bw + nx;
-        const uint32_t nm = P[n] & 15u;
+        const uint32_t vn = P[n];
+        const uint32_t nm = vn & 15u;
         const bool adj = k == 0 ? adj_left(m, nm) : (k == 1 ? adj_upleft(m, nm) : (k == 2 ? adj_up(m, nm) : adj_upright(m, nm)));
-        if (adj) Forest::unite(P, b, n);
+        if (adj) Forest::unite(P, vb >> 4, vn >> 4);   // tile-local roots
     }
 }
 
@@ -237,9 +239,9 @@ __global__ __launch_bounds__(256) void bleq2d_flatten(uint32_t *P, uint32_t bw, 
     uint32_t bx, by;
     border_block(j, bx0, by0, bx, by);
     if (bx >= bw || by >= bh) return;
-    const uint32_t v = P[by * bw + bx];
-    if (!(v & 15u)) return;
-    Forest::compress(P, v >> 4);   // the tile-local root of this block
+    const uint32_t b = by * bw + bx;
+    if (!(P[b] & 15u)) return;
+    Forest::compress(P, b);
 }
 
 // ---- 4. final kernel -----------------------------------------------------------

@@ -75,6 +75,10 @@ __device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b)
 // ---- global forest with a packed low field ---------------------------------
 // word = parent << SHIFT | low, where `low` (the block's foreground mask) never
 // changes.  SHIFT = 0 with an all-zero low field is the plain parent array.
+//
+// Callers start find/unite at tile-local roots (one hop from any element, as
+// the tile kernels leave it), so path halving only ever rewrites root entries:
+// an ordinary element keeps pointing at its tile-local root until compress.
 
 template <int SHIFT>
 struct PackedForest {
@@ -123,13 +127,17 @@ struct PackedForest {
         }
     }
 
-    // After all unions: point x's entry straight at its root.
+    // After all unions: point element x and its tile-local root straight at
+    // the final root (every write is an atomicMin towards that root).
     __device__ static void compress(uint32_t *w, uint32_t x)
     {
-        uint32_t v = dev_load(&w[x]);
-        if ((v >> SHIFT) == x) return;
-        uint32_t r = find(w, x);
-        dev_min(&w[x], (r << SHIFT) | (v & kLowMask));
+        const uint32_t vx = dev_load(&w[x]);
+        const uint32_t r = vx >> SHIFT;
+        const uint32_t vr = dev_load(&w[r]);
+        if ((vr >> SHIFT) == r) return;          // already one hop from the root
+        const uint32_t root = find(w, r);
+        dev_min(&w[r], (root << SHIFT) | (vr & kLowMask));
+        if (r != x) dev_min(&w[x], (root << SHIFT) | (vx & kLowMask));
     }
 };
 
@@ -166,12 +174,15 @@ struct LabelForest {
         }
     }
 
+    // As PackedForest::compress: x and its tile-local root -> final root.
     __device__ static void compress(uint32_t *L, uint32_t x)
     {
-        uint32_t v = dev_load(&L[x]);
-        if (v - 1u == x) return;
-        uint32_t r = find(L, x);
-        dev_min(&L[x], r + 1u);
+        const uint32_t r = dev_load(&L[x]) - 1u;
+        const uint32_t vr = dev_load(&L[r]);
+        if (vr - 1u == r) return;
+        const uint32_t root = find(L, r);
+        dev_min(&L[r], root + 1u);
+        if (r != x) dev_min(&L[x], root + 1u);
     }
 };
 

@@ -113,19 +113,22 @@ def test_host_pitch(labeler):
 
 def test_device_entry_point_unaligned_pitch(labeler):
     """Device buffers with a pitch that is not a multiple of 16 (scalar load path)."""
-    torch = pytest.importorskip("torch")
+    from tests import hiprt
+
     h, w, pitch = 257, 1001, 1003
     host = L.bernoulli((h, pitch), 0.45, seed=21)
-    d_px = torch.from_numpy(host).cuda()
-    d_lab = torch.empty((h, w), dtype=torch.int32, device="cuda")
-    d_n = torch.zeros(1, dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-    labeler.label2d_dev(d_px.data_ptr(), h, w, pitch, d_lab.data_ptr(), "bleq", 0, d_n.data_ptr(), stream)
-    torch.cuda.synchronize()
-    lab = d_lab.cpu().numpy().view(np.uint32)
+    d_px = hiprt.to_device(host)
+    d_lab = hiprt.DeviceBuffer(h * w * 4 + 4)
+    d_n = hiprt.DeviceBuffer(4)
+    stream = hiprt.Stream()
+    # labels at a 4-byte (not 16-byte) aligned address: scalar store path
+    labeler.label2d_dev(d_px.addr, h, w, pitch, d_lab.addr + 4, "bleq", 0, d_n.addr, stream.handle.value)
+    stream.synchronize()
+    lab = d_lab.download((h, w), np.uint32, offset=4)
     canon, m = O.bfs2d(np.ascontiguousarray(host[:, :w]), 8)
     np.testing.assert_array_equal(O.canonical(lab)[0], canon)
-    assert int(d_n.item()) == m
+    assert int(d_n.download((1,), np.uint32)[0]) == m
+    stream.destroy()
 
 
 def test_errors(labeler):
