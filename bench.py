@@ -1,20 +1,25 @@
 #!/usr/bin/env python3
-"""Throughput of 2D block-equivalence labeling on one MI355X.
+"""Throughput of block-equivalence labeling on one MI355X.
 
-Workload: 8192x8192 uint8 binary images, the six inputs of the single-image
-configuration (i.i.d. Bernoulli p in {0.1, 0.3, 0.5, 0.7, 0.9} and Gaussian-
-smoothed noise blobs, sigma 4 px, 50% quantile), 8-connectivity, uint32
-labels.  One step = one full labeling of one image (inputs cycle), inputs and
-outputs resident in HBM, no host synchronisation inside the timed region.
+Headline (the JSON line's value): 2D block equivalence on 8192x8192 uint8
+binary images -- the six inputs of the single-image configuration (i.i.d.
+Bernoulli p in {0.1, 0.3, 0.5, 0.7, 0.9} and Gaussian-smoothed noise blobs,
+sigma 4 px, 50% quantile), 8-connectivity, uint32 labels.  One step = one
+labeling of one image (inputs cycle), inputs and outputs resident in HBM, no
+host synchronisation inside the timed region.  value = Gpixel/s.
 
-Prints one JSON line: value = Gpixel/s over the timed steps; `roofline` for the
-labeling pipeline (algorithmic 5 B per pixel: 1 B in + 4 B label out, against
-8 TB/s), measured with HIP events on the stream the kernels run on; and
-`cpu_baseline`: the oracle's OpenMP restatement of the reference CPU labeler
-(TLabelEquivalenceX2) timed on this host.
+`roofline` covers the labeling pipeline of one image (all its kernels):
+algorithmic 5 B per pixel (1 B in + 4 B label out) over the summed device time
+of the kernels, measured with HIP events that the library records between its
+kernels on the launch stream; peak 8 TB/s.  `cpu_baseline` is the oracle's
+OpenMP restatement of the reference CPU labeler (TLabelEquivalenceX2) on this
+host.  `extra` holds the other configurations: 3D 512^3, label equivalence,
+the batched 4096-image workload (one hipGraph launch vs per-image launches)
+and the continuous-batching server (p50/p99 latency at a fixed request rate).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -23,7 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_GBS = 8000.0
-ALGO_BYTES_PER_PX = 5.0
+ALGO_BYTES = 5.0
 METRIC = "Gpixel/s 2D 8192², Gvoxel/s 3D 512³, batched vs per-image; % HBM roofline"
 
 
@@ -31,25 +36,275 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_inputs(side):
-    import labeling_amd as L
+def timed_steps(torch, stream, step, n_inputs, warmup, steps):
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(steps):
+        step(i)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1), time.perf_counter() - t0
 
-    out = []
-    for p in (0.1, 0.3, 0.5, 0.7, 0.9):
-        out.append((f"bern_p{p}", L.bernoulli((side, side), p, seed=0)))
-    out.append(("blob_s4", L.blobs2d(side, side, 4.0, 0.5, seed=0)))
-    return out
+
+def kernel_breakdown(lab, step, names, reps):
+    """Average per-kernel device time (us) over reps steps, and per-input pipeline time."""
+    lab.set_profiling(True)
+    kernels, per_input = {}, {}
+    for i in range(reps):
+        step(i)
+        kt = lab.kernel_times()
+        for name, us in kt:
+            kernels.setdefault(name, []).append(us)
+        per_input.setdefault(names[i % len(names)], []).append(sum(us for _, us in kt))
+    lab.set_profiling(False)
+    avg = {k: sum(v) / len(v) for k, v in kernels.items()}
+    return avg, {k: sum(v) / len(v) for k, v in per_input.items()}
 
 
-def traffic_from_profiles():
-    path = os.path.join(ROOT, "profiles", "bleq2d_8192_traffic.json")
-    if not os.path.exists(path):
-        return None
+def roofline(units, path_us, traffic=None, scope=""):
+    achieved = ALGO_BYTES * units / (path_us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic, "scope": scope}
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", name)
     try:
-        return json.load(open(path)).get("hbm_bytes_per_image")
+        return json.load(open(path)).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
 
+
+# ---- 2D single image -----------------------------------------------------------
+
+def bench_2d(L, torch, lab, a):
+    side = a.side
+    log(f"[bench] 2D inputs {side}x{side}")
+    inputs = [(f"bern_p{p}", L.bernoulli((side, side), p, seed=0)) for p in (0.1, 0.3, 0.5, 0.7, 0.9)]
+    inputs.append(("blob_s4", L.blobs2d(side, side, 4.0, 0.5, seed=0)))
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(img).to(dev) for _, img in inputs]
+    d_lab = torch.empty((side, side), dtype=torch.int32, device=dev)
+    d_n = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    def step(i, alg="bleq", conn=0):
+        x = d_in[i % len(d_in)]
+        lab.label2d_dev(x.data_ptr(), side, side, side, d_lab.data_ptr(), alg, conn, d_n.data_ptr(), sptr)
+
+    log(f"[bench] 2D timed {a.steps} steps")
+    ms, wall = timed_steps(torch, stream, step, len(inputs), a.warmup, a.steps)
+    names = [n for n, _ in inputs]
+    kavg, per_input = kernel_breakdown(lab, step, names, max(len(inputs), min(a.steps, 30)))
+    path_us = sum(kavg.values())
+    px = side * side
+    rl = roofline(px, path_us, traffic_from_profiles("bleq2d_8192_traffic.json"),
+                  "bleq 2D labeling pipeline, one image (all kernels), algorithmic 5 B/px")
+    rl["path_us"] = round(path_us, 2)
+    rl["kernels_us"] = {k: round(v, 2) for k, v in kavg.items()}
+    rl["per_input_us"] = {k: round(v, 2) for k, v in per_input.items()}
+    out = {"value": px * a.steps / (ms / 1e3) / 1e9, "ms_per_step": ms / a.steps, "wall_s": wall,
+           "roofline": rl, "inputs": inputs}
+
+    if not a.quick:
+        log("[bench] lbeq 2D")
+        lbeq = {}
+        for conn in (4, 8):
+            ms2, _ = timed_steps(torch, stream, lambda i: step(i, "lbeq", conn), len(inputs), 2, 12)
+            lbeq[f"conn{conn}_gpx_s"] = round(px * 12 / (ms2 / 1e3) / 1e9, 2)
+        out["lbeq2d"] = lbeq
+    del d_in
+    return out
+
+
+# ---- 3D ---------------------------------------------------------------------------
+
+def bench_3d(L, torch, lab, a):
+    s = a.side3d
+    log(f"[bench] 3D inputs {s}^3")
+    vols = [("bern_p0.1", L.bernoulli((s, s, s), 0.1, seed=0)), ("bern_p0.3", L.bernoulli((s, s, s), 0.3, seed=0)),
+            ("blob_s3", L.blobs3d(s, s, s, 3.0, 0.4, seed=0))]
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(v).to(dev) for _, v in vols]
+    d_lab = torch.empty((s, s, s), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+    res = {"workload": f"{s}^3 uint8 volumes (Bernoulli p=0.1, 0.3; blobs sigma=3 q=0.4), 26-connectivity"}
+    for alg in ("bleq", "lbeq"):
+        def step(i, alg=alg):
+            x = d_in[i % len(d_in)]
+            lab.label3d_dev(x.data_ptr(), s, s, s, d_lab.data_ptr(), alg, 0, None, sptr)
+
+        steps = 30 if alg == "bleq" else 9
+        ms, _ = timed_steps(torch, stream, step, len(vols), 3, steps)
+        kavg, per_input = kernel_breakdown(lab, step, [n for n, _ in vols], 9)
+        path_us = sum(kavg.values())
+        r = {"gvox_s": round(s ** 3 * steps / (ms / 1e3) / 1e9, 2), "ms_per_step": round(ms / steps, 4),
+             "roofline": roofline(s ** 3, path_us, None, f"{alg} 3D pipeline, one volume, algorithmic 5 B/voxel")}
+        r["roofline"]["path_us"] = round(path_us, 2)
+        r["roofline"]["kernels_us"] = {k: round(v, 2) for k, v in kavg.items()}
+        r["per_input_us"] = {k: round(v, 2) for k, v in per_input.items()}
+        res[alg] = r
+    del d_in
+    return res
+
+
+# ---- batched ------------------------------------------------------------------------
+
+def bench_batch(L, torch, lab, a):
+    import ctypes
+
+    import numpy as np
+
+    n = a.batch
+    specs = L.batched_workload(n, seed=0)
+    total_px = sum(h * w for h, w, _ in specs)
+    log(f"[bench] batch {n} images, {total_px / 1e9:.3f} Gpx")
+    dev = torch.device("cuda:0")
+    # inputs: one buffer, 16-byte pitched rows per image; labels: one buffer, dense per image
+    in_off, lab_off, o_in, o_lab = [], [], 0, 0
+    for h, w, _ in specs:
+        in_off.append(o_in)
+        lab_off.append(o_lab)
+        o_in += h * ((w + 15) // 16 * 16)
+        o_lab += (h * w + 3) // 4 * 4
+    d_in = torch.empty(o_in, dtype=torch.uint8, device=dev)
+    d_lab = torch.empty(o_lab, dtype=torch.int32, device=dev)
+    d_n = torch.zeros(n, dtype=torch.int32, device=dev)
+    for i, (h, w, p) in enumerate(specs):
+        pitch = (w + 15) // 16 * 16
+        img = np.zeros((h, pitch), np.uint8)
+        img[:, :w] = L.bernoulli((h, w), p, seed=0, stream=i)
+        d_in[in_off[i]:in_off[i] + h * pitch].copy_(torch.from_numpy(img.reshape(-1)))
+    torch.cuda.synchronize()
+    descs = [(d_in.data_ptr() + in_off[i], d_lab.data_ptr() + 4 * lab_off[i], h, w, (w + 15) // 16 * 16)
+             for i, (h, w, _) in enumerate(specs)]
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    log("[bench] batch autotune + graph")
+    t0 = time.perf_counter()
+    batch = L.Batch(lab, descs, "bleq", 0, d_n.data_ptr(), L.BATCH_AUTOTUNE)
+    plan_s = time.perf_counter() - t0
+    reps = 5
+    ms_b, _ = timed_steps(torch, stream, lambda i: batch.run(sptr), 1, 1, reps)
+    batch_s = ms_b / reps / 1e3
+
+    log("[bench] per-image launches")
+    arr = (L.ImageDesc * n)()
+    for i, d in enumerate(descs):
+        arr[i] = L.ImageDesc(*d)
+    lib = lab.lib
+
+    def each(_i):
+        st = lib.lbl_label2d_each(lab.handle, L.BLEQ, arr, n, 0, ctypes.c_void_p(d_n.data_ptr()),
+                                  ctypes.c_void_p(sptr))
+        if st != L.OK:
+            raise L.LabelingError(st, lib.lbl_last_error(lab.handle).decode())
+
+    ms_e, _ = timed_steps(torch, stream, each, 1, 1, 2)
+    each_s = ms_e / 2 / 1e3
+    cfg = batch.config()
+    batch.close()
+    del d_in, d_lab
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{n} uint8 images, H,W ~ U[64,2048], p ~ U[0.05,0.95] per image, 8-connectivity",
+        "pixels": total_px,
+        "batched_images_s": round(n / batch_s, 1),
+        "batched_gpx_s": round(total_px / batch_s / 1e9, 2),
+        "batched_ms": round(batch_s * 1e3, 3),
+        "per_image_images_s": round(n / each_s, 1),
+        "per_image_ms": round(each_s * 1e3, 3),
+        "speedup": round(each_s / batch_s, 2),
+        "roofline_frac": round(ALGO_BYTES * total_px / batch_s / 1e9 / PEAK_GBS, 4),
+        "plan_and_autotune_s": round(plan_s, 2),
+        "config": cfg,
+    }
+
+
+# ---- serving ---------------------------------------------------------------------------
+
+def bench_serve(L, lab, a):
+    import numpy as np
+
+    rng_seed = 7
+    pool = []
+    log("[bench] serving request pool")
+    for i in range(a.serve_pool):
+        u = L.synth_u64(rng_seed, 0x5A, i)
+        blob = (u >> 8) & 1
+        if u % 10 < 8:
+            side = 256 + L.synth_u64(rng_seed, 0x5B, i) % (4096 - 256 + 1)
+            shape = (side, side)
+            img = L.blobs2d(side, side, 4.0, 0.5, seed=i) if blob else \
+                L.bernoulli(shape, 0.05 + 0.9 * (u % 1000) / 1000, rng_seed, stream=i)
+        else:
+            side = 64 + L.synth_u64(rng_seed, 0x5B, i) % (256 - 64 + 1)
+            shape = (side, side, side)
+            img = L.blobs3d(side, side, side, 3.0, 0.4, seed=i) if blob else \
+                L.bernoulli(shape, 0.05 + 0.3 * (u % 1000) / 1000, rng_seed, stream=i)
+        pin = L.PinnedArray(shape, np.uint8)
+        pin.array[...] = img
+        pool.append(pin)
+    max_elems = max(p.array.size for p in pool)
+    outs = [L.PinnedArray((max_elems,), np.uint32) for _ in range(a.serve_outputs)]
+    srv = L.Server(lab, streams=a.serve_streams, hbm_budget_bytes=a.serve_budget << 20, max_batch=16)
+    qps, dur = a.serve_qps, a.serve_seconds
+    log(f"[bench] serving {qps} req/s for {dur} s")
+    free_out = list(range(len(outs)))
+    ticket_out = {}
+    lat, done_n, dropped, submitted = [], 0, 0, 0
+    arrivals = []
+    t = 0.0
+    k = 0
+    while t < dur:   # Poisson arrivals: exponential gaps from the counter-based generator
+        u = (L.synth_u64(rng_seed, 0x5C, k) >> 11) / float(1 << 53)
+        t += -math.log(1.0 - u) / qps
+        arrivals.append(t)
+        k += 1
+    t_start = srv.now()
+    i = 0
+    while i < len(arrivals) or ticket_out:
+        now = srv.now() - t_start
+        while i < len(arrivals) and arrivals[i] <= now:
+            if not free_out:
+                dropped += 1   # no output buffer free: counted, not submitted
+            else:
+                req = pool[i % len(pool)]
+                o = free_out.pop()
+                tk = srv.submit(req.ptr, outs[o].ptr, req.shape, "bleq", 0)
+                ticket_out[tk] = o
+                submitted += 1
+            i += 1
+        for tk, st, ncomp, t_sub, t_beg, t_done in srv.poll():
+            free_out.append(ticket_out.pop(tk))
+            lat.append(t_done - t_sub)
+            done_n += 1
+        time.sleep(0.0002)
+    wall = srv.now() - t_start
+    srv.close()
+    for p in pool + outs:
+        p.free()
+    lat.sort()
+    q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))] * 1e3 if lat else None  # noqa: E731
+    return {
+        "workload": "Poisson arrivals; 80% 2D (side U[256,4096]), 20% 3D (side U[64,256]); half Bernoulli, half "
+                    "blobs; host pinned buffers in/out (latency includes PCIe copies); bleq",
+        "qps_target": qps, "seconds": dur, "submitted": submitted, "completed": done_n, "dropped": dropped,
+        "throughput_req_s": round(done_n / wall, 1),
+        "latency_ms": {"p50": q(0.50), "p90": q(0.90), "p99": q(0.99), "max": q(1.0)},
+        "streams": a.serve_streams, "budget_mib": a.serve_budget,
+    }
+
+
+# ---- CPU baseline ----------------------------------------------------------------------------
 
 def cpu_baseline(inputs, seconds_budget):
     from oracle import pyoracle as O
@@ -57,22 +312,15 @@ def cpu_baseline(inputs, seconds_budget):
     threads = min(16, os.cpu_count() or 1)
     total_px, total_s, runs = 0, 0.0, 0
     t_start = time.perf_counter()
-    while time.perf_counter() - t_start < seconds_budget or runs < len(inputs):
+    while (time.perf_counter() - t_start < seconds_budget or runs < len(inputs)) and runs < 6 * len(inputs):
         name, img = inputs[runs % len(inputs)]
         _, info = O.bleq2d(img, threads=threads, with_info=True)
         total_px += img.size
         total_s += info["seconds"]
         runs += 1
-        if runs >= 6 * len(inputs):
-            break
-    return {
-        "value": round(total_px / total_s / 1e9, 4),
-        "unit": "Gpixel/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{runs} labelings of the 8192x8192 inputs (cycled), oracle OpenMP restatement of "
-                  f"TLabelEquivalenceX2 on the padded image, DoLabel region only",
-    }
+    return {"value": round(total_px / total_s / 1e9, 4), "unit": "Gpixel/s", "cores": threads, "kind": "port",
+            "sample": f"{runs} labelings of the 8192x8192 inputs (cycled), oracle OpenMP restatement of "
+                      f"TLabelEquivalenceX2 on the padded image, DoLabel region only"}
 
 
 def main():
@@ -81,111 +329,77 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--side", type=int, default=8192)
+    ap.add_argument("--side3d", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-kernel-times", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline 2D line only")
+    ap.add_argument("--skip", default="", help="comma list of extras to skip: 3d,batch,serve")
+    ap.add_argument("--serve-qps", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-budget", type=int, default=4096, help="MiB")
+    ap.add_argument("--serve-pool", type=int, default=24)
+    ap.add_argument("--serve-outputs", type=int, default=16)
+    ap.add_argument("--calib-copy", type=int, default=0, help="MiB: only run a plain device copy (counter scale)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (a single image or batch is "
-              f"labelled on one device; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (images are labelled on one "
+              f"device; DESIGN.md describes how the path would shard)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    import torch   # initialise torch's HIP runtime before the library binds to it
 
+    torch.zeros(1, device="cuda:0")
+    if a.calib_copy:
+        # plain device copy of a known size: the reference point that scales the
+        # FETCH_SIZE / WRITE_SIZE counters into HBM bytes
+        src = torch.ones(a.calib_copy << 20, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": a.calib_copy << 20, "launches": 5}), flush=True)
+        return
     import labeling_amd as L
 
-    side = a.side
-    log(f"[bench] generating inputs {side}x{side}")
-    inputs = make_inputs(side)
-    dev = torch.device("cuda:0")
-    d_inputs = [torch.from_numpy(img).to(dev) for _, img in inputs]
-    d_labels = torch.empty((side, side), dtype=torch.int32, device=dev)
-    d_count = torch.zeros(1, dtype=torch.int32, device=dev)
     lab = L.Labeler(0)
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
-
-    def step(i):
-        x = d_inputs[i % len(d_inputs)]
-        lab.label2d_dev(x.data_ptr(), side, side, side, d_labels.data_ptr(), "bleq", 0, d_count.data_ptr(), sptr)
-
-    log(f"[bench] warmup {a.warmup}")
-    for i in range(a.warmup):
-        step(i)
-    torch.cuda.synchronize()
-
-    log(f"[bench] timed {a.steps} steps")
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for i in range(a.steps):
-        step(i)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = ev0.elapsed_time(ev1)
-    ms_per_step = ms_total / a.steps
-    px = side * side
-    value = px * a.steps / (ms_total / 1e3) / 1e9
-
-    # per-kernel device times (separate pass; events between the kernels)
-    kernels = {}
-    per_input = {}
-    if not a.no_kernel_times:
-        lab.set_profiling(True)
-        reps = max(1, min(a.steps, 30))
-        for i in range(reps):
-            step(i)
-            kt = lab.kernel_times()
-            for name, us in kt:
-                kernels.setdefault(name, []).append(us)
-            per_input.setdefault(inputs[i % len(inputs)][0], []).append(sum(us for _, us in kt))
-        lab.set_profiling(False)
-    kernel_avg = {k: sum(v) / len(v) for k, v in kernels.items()}
-    path_us = sum(kernel_avg.values()) if kernel_avg else ms_per_step * 1e3
-    achieved = ALGO_BYTES_PER_PX * px / (path_us * 1e-6) / 1e9
-    traffic = traffic_from_profiles()
-
-    cpu = None
-    if not a.no_cpu_baseline:
-        log("[bench] cpu baseline")
-        cpu = cpu_baseline(inputs, a.cpu_seconds)
-
+    r2 = bench_2d(L, torch, lab, a)
+    skip = set(filter(None, a.skip.split(",")))
+    extra = {}
+    if not a.quick:
+        if "lbeq2d" in r2:
+            extra["lbeq2d_8192"] = r2.pop("lbeq2d")
+        if "3d" not in skip:
+            extra["volume_512"] = bench_3d(L, torch, lab, a)
+        if "batch" not in skip:
+            extra["batched_4096"] = bench_batch(L, torch, lab, a)
+        if "serve" not in skip:
+            extra["serving"] = bench_serve(L, lab, a)
+    cpu = None if a.no_cpu_baseline else cpu_baseline(r2["inputs"], a.cpu_seconds)
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(r2["value"], 3),
         "unit": "Gpixel/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step": round(r2["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
         "config": {
-            "workload": f"bleq 2D single image {side}x{side} uint8, 8-connectivity, uint32 labels; inputs cycle "
+            "workload": f"bleq 2D single image {a.side}x{a.side} uint8, 8-connectivity, uint32 labels; inputs cycle "
                         "Bernoulli p=0.1,0.3,0.5,0.7,0.9 and blobs sigma=4 q=0.5",
             "global_batch": 1,
-            "pixels_per_step": px,
-            "wall_s": round(wall, 4),
+            "pixels_per_step": a.side * a.side,
+            "wall_s": round(r2["wall_s"], 4),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / PEAK_GBS, 4),
-            "traffic": traffic,
-            "scope": "labeling pipeline (all kernels of one image), algorithmic 5 B/px",
-            "path_us": round(path_us, 2),
-            "kernels_us": {k: round(v, 2) for k, v in kernel_avg.items()},
-            "per_input_us": {k: round(sum(v) / len(v), 2) for k, v in per_input.items()},
-        },
+        "roofline": r2["roofline"],
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
     lab.close()

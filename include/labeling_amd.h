@@ -142,6 +142,114 @@ lbl_status lbl_label3d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_vox, uint
 lbl_status lbl_set_profiling(lbl_ctx *ctx, int enable);
 int lbl_last_kernel_times(lbl_ctx *ctx, float *us, const char **names, int cap);
 
+/* ---- batches of 2D images ------------------------------------------------------ */
+
+/* One device-resident 2D image of a batch. */
+typedef struct {
+    const uint8_t *px;      /* device, h rows of pitch_bytes */
+    uint32_t *labels;       /* device, h*w dense */
+    uint32_t h, w;
+    size_t pitch_bytes;
+} lbl_image2d;
+
+typedef struct lbl_batch lbl_batch;
+
+enum {
+    LBL_BATCH_AUTOTUNE = 1,   /* time the tile shapes and merge schedules on this batch, keep the fastest */
+    LBL_BATCH_NO_GRAPH = 2    /* lbl_batch_run launches the kernels directly instead of replaying a hipGraph */
+};
+
+/* Prepares n images of uneven sizes for labeling as one unit: a device table of
+ * per-image geometry, one workspace, the launch configuration (autotuned per
+ * size class, or from the context's tuning table), and a hipGraph holding the
+ * four batched kernels.  No reference counterpart: the reference labels a
+ * directory with a sequential loop of Label() calls (ProcessImages,
+ * main.cpp:309-340), which is what one lbl_label2d_dev call per image mirrors.
+ * Each image is labelled exactly as lbl_label2d_dev would label it alone.
+ * d_ncomp (nullable, device): n per-image component counts.  The device
+ * buffers must stay valid while the batch exists.  bleq only (LBL_EUNSUPPORTED
+ * for lbeq). */
+lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_image2d *images, uint32_t n, int conn,
+                            uint32_t *d_ncomp, unsigned flags, lbl_batch **out);
+
+/* Labels every image of the batch: one hipGraph launch on `stream`. */
+lbl_status lbl_batch_run(lbl_batch *b, void *stream);
+
+/* Launch configuration chosen for the batch: tile shape id and size in 2x2
+ * blocks, whether the flatten pass runs. */
+lbl_status lbl_batch_config(const lbl_batch *b, uint32_t *tile_id, uint32_t *tile_w, uint32_t *tile_h,
+                            int *flatten);
+
+void lbl_batch_destroy(lbl_batch *b);
+
+/* Labels n device images one after another with their own per-image launch
+ * sequences (what n lbl_label2d_dev calls do, without per-call overhead in the
+ * caller): the device-side form of the reference's sequential directory loop
+ * (ProcessImages, main.cpp:309-340), and the baseline a batch is measured
+ * against.  d_ncomp (nullable, device): n counts. */
+lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_image2d *images, uint32_t n, int conn,
+                            uint32_t *d_ncomp, void *stream);
+
+/* ---- autotuning ----------------------------------------------------------------- */
+
+/* Times every tile shape and merge schedule of the 2D path on one device image
+ * and records the fastest for images of its size class (floor(log2(h*w))) in
+ * the context's tuning table, which later 2D calls of that class use. */
+lbl_status lbl_autotune2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
+                          size_t pitch_bytes, uint32_t *d_labels, void *stream);
+
+/* Tuning table as JSON ({"<alg>/<kind>/<size class>": [tile, flatten, us], ...}). */
+lbl_status lbl_tuning_save(lbl_ctx *ctx, const char *path);
+lbl_status lbl_tuning_load(lbl_ctx *ctx, const char *path);
+
+/* ---- serving ------------------------------------------------------------------------
+ * No reference counterpart (the reference labels one image per call).  A server
+ * owns a device arena of `hbm_budget_bytes` and `streams` device streams, and a
+ * scheduler thread that does continuous batching: requests that arrive while
+ * others are on the GPU are admitted as soon as a stream is free and their
+ * input, labels and union-find workspace fit in the arena; consecutive 2D bleq
+ * requests are fused into one batched launch.  Each result is identical to the
+ * single-request entry points' result on the same input. */
+
+typedef struct lbl_server lbl_server;
+
+typedef struct {
+    uint32_t streams;            /* concurrent device streams (1..16) */
+    uint64_t hbm_budget_bytes;   /* device arena for in-flight requests */
+    uint32_t max_batch;          /* most 2D requests fused into one launch (>= 1) */
+} lbl_server_config;
+
+typedef struct {
+    lbl_alg alg;
+    int conn;                    /* as for lbl_label2d / lbl_label3d */
+    uint32_t d, h, w;            /* d = 0 for a 2D image */
+    const uint8_t *px;           /* host input, dense; pinned memory (lbl_host_alloc) lets copies overlap */
+    uint32_t *labels;            /* host output, dense */
+} lbl_request;
+
+typedef struct {
+    uint64_t ticket;
+    lbl_status status;
+    uint32_t n_components;
+    double t_submit, t_start, t_done;   /* seconds on the server's steady clock */
+} lbl_completion;
+
+lbl_status lbl_server_create(lbl_ctx *ctx, const lbl_server_config *cfg, lbl_server **out);
+/* Queues a request (never blocks on the device).  LBL_ENOMEM when the request
+ * alone exceeds the arena; the buffers must stay valid until it completes. */
+lbl_status lbl_server_submit(lbl_server *srv, const lbl_request *req, uint64_t *ticket);
+/* Moves up to `cap` finished requests into out[]; *n receives the count. */
+lbl_status lbl_server_poll(lbl_server *srv, lbl_completion *out, uint32_t cap, uint32_t *n);
+/* Blocks until every submitted request has finished. */
+lbl_status lbl_server_drain(lbl_server *srv);
+/* Seconds on the server's clock (the clock of lbl_completion). */
+double lbl_server_now(const lbl_server *srv);
+void lbl_server_destroy(lbl_server *srv);
+
+/* Page-locked host buffers for requests and results. */
+void *lbl_host_alloc(size_t bytes);
+void lbl_host_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,32 @@ _lib = None
 _synth = None
 
 
+BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+
+
+class ImageDesc(ctypes.Structure):
+    """lbl_image2d"""
+    _fields_ = [("px", ctypes.c_void_p), ("labels", ctypes.c_void_p), ("h", ctypes.c_uint32),
+                ("w", ctypes.c_uint32), ("pitch_bytes", ctypes.c_size_t)]
+
+
+class ServerConfig(ctypes.Structure):
+    """lbl_server_config"""
+    _fields_ = [("streams", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    """lbl_request"""
+    _fields_ = [("alg", ctypes.c_int), ("conn", ctypes.c_int), ("d", ctypes.c_uint32), ("h", ctypes.c_uint32),
+                ("w", ctypes.c_uint32), ("px", ctypes.c_void_p), ("labels", ctypes.c_void_p)]
+
+
+class Completion(ctypes.Structure):
+    """lbl_completion"""
+    _fields_ = [("ticket", ctypes.c_uint64), ("status", ctypes.c_int), ("n_components", ctypes.c_uint32),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
 class LabelingError(RuntimeError):
     """A non-OK lbl_status, with the context's last error message."""
 
@@ -64,6 +90,23 @@ def _declare(lib):
         "lbl_label3d_dev": (i32, [P, i32, P, u32, u32, u32, i32, P, P, P]),
         "lbl_set_profiling": (i32, [P, i32]),
         "lbl_last_kernel_times": (i32, [P, pf, ctypes.POINTER(ctypes.c_char_p), i32]),
+        "lbl_batch_create": (i32, [P, i32, ctypes.POINTER(ImageDesc), u32, i32, P, ctypes.c_uint,
+                                   ctypes.POINTER(P)]),
+        "lbl_batch_run": (i32, [P, P]),
+        "lbl_batch_config": (i32, [P, pu32, pu32, pu32, ctypes.POINTER(i32)]),
+        "lbl_batch_destroy": (None, [P]),
+        "lbl_label2d_each": (i32, [P, i32, ctypes.POINTER(ImageDesc), u32, i32, P, P]),
+        "lbl_autotune2d": (i32, [P, i32, P, u32, u32, sz, P, P]),
+        "lbl_tuning_save": (i32, [P, ctypes.c_char_p]),
+        "lbl_tuning_load": (i32, [P, ctypes.c_char_p]),
+        "lbl_server_create": (i32, [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
+        "lbl_server_submit": (i32, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+        "lbl_server_poll": (i32, [P, ctypes.POINTER(Completion), u32, pu32]),
+        "lbl_server_drain": (i32, [P]),
+        "lbl_server_now": (ctypes.c_double, [P]),
+        "lbl_server_destroy": (None, [P]),
+        "lbl_host_alloc": (P, [sz]),
+        "lbl_host_free": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -226,6 +269,132 @@ class Labeler:
                     d_ncomp: Optional[int] = None, stream: Optional[int] = None):
         self._check(self.lib.lbl_label3d_dev(self._ctx, _alg_id(alg), d_vox, d, h, w, conn, d_labels,
                                              d_ncomp or None, stream or None))
+
+
+class Batch:
+    """lbl_batch: device images labelled together by one hipGraph launch."""
+
+    def __init__(self, labeler: Labeler, images, alg="bleq", conn: int = 0, d_ncomp: Optional[int] = None,
+                 flags: int = 0):
+        """images: sequence of (d_px, d_labels, h, w, pitch_bytes) device descriptors."""
+        self.labeler = labeler
+        lib = labeler.lib
+        n = len(images)
+        descs = (ImageDesc * n)()
+        for i, (px, lab, h, w, pitch) in enumerate(images):
+            descs[i] = ImageDesc(px, lab, h, w, pitch)
+        self._b = ctypes.c_void_p()
+        labeler._check(lib.lbl_batch_create(labeler.handle, _alg_id(alg), descs, n, conn, d_ncomp or None, flags,
+                                            ctypes.byref(self._b)))
+
+    def run(self, stream: Optional[int] = None):
+        self.labeler._check(self.labeler.lib.lbl_batch_run(self._b, stream or None))
+
+    def config(self):
+        t, tw, th, fl = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int()
+        self.labeler.lib.lbl_batch_config(self._b, ctypes.byref(t), ctypes.byref(tw), ctypes.byref(th),
+                                          ctypes.byref(fl))
+        return {"tile_id": t.value, "tile_blocks": [tw.value, th.value], "flatten": bool(fl.value)}
+
+    def close(self):
+        if self._b:
+            self.labeler.lib.lbl_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """numpy view of page-locked host memory (lbl_host_alloc)."""
+
+    def __init__(self, shape, dtype):
+        self.lib = load()
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = self.lib.lbl_host_alloc(max(nbytes, 1))
+        if not self.ptr:
+            raise MemoryError("lbl_host_alloc failed")
+        buf = (ctypes.c_uint8 * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(self.dtype).reshape(self.shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            self.lib.lbl_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """lbl_server: continuous-batching request server."""
+
+    def __init__(self, labeler: Labeler, streams: int = 4, hbm_budget_bytes: int = 8 << 30, max_batch: int = 32):
+        self.labeler = labeler
+        self.lib = labeler.lib
+        cfg = ServerConfig(streams, hbm_budget_bytes, max_batch)
+        self._s = ctypes.c_void_p()
+        labeler._check(self.lib.lbl_server_create(labeler.handle, ctypes.byref(cfg), ctypes.byref(self._s)))
+
+    def submit(self, px_ptr: int, labels_ptr: int, shape, alg="bleq", conn: int = 0) -> int:
+        if len(shape) == 2:
+            d, (h, w) = 0, shape
+        else:
+            d, h, w = shape
+        req = Request(_alg_id(alg), conn, d, h, w, px_ptr, labels_ptr)
+        t = ctypes.c_uint64()
+        st = self.lib.lbl_server_submit(self._s, ctypes.byref(req), ctypes.byref(t))
+        if st != OK:
+            raise LabelingError(st, "lbl_server_submit rejected the request")
+        return t.value
+
+    def poll(self, cap: int = 1024):
+        out = (Completion * cap)()
+        n = ctypes.c_uint32()
+        st = self.lib.lbl_server_poll(self._s, out, cap, ctypes.byref(n))
+        if st != OK:
+            raise LabelingError(st, "server device failure")
+        return [(c.ticket, c.status, c.n_components, c.t_submit, c.t_start, c.t_done) for c in out[:n.value]]
+
+    def drain(self):
+        st = self.lib.lbl_server_drain(self._s)
+        if st != OK:
+            raise LabelingError(st, "server device failure")
+
+    def now(self) -> float:
+        return float(self.lib.lbl_server_now(self._s))
+
+    def close(self):
+        if self._s:
+            self.lib.lbl_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batched_workload(n: int, seed: int, lo: int = 64, hi: int = 2048):
+    """Shapes and densities of the batched configuration: H, W ~ U{lo..hi} and
+    p ~ U[0.05, 0.95] independently per image, from (seed, image index)."""
+    span = hi - lo + 1
+    out = []
+    for i in range(n):
+        h = lo + synth_u64(seed, 0xB, 3 * i) % span
+        w = lo + synth_u64(seed, 0xB, 3 * i + 1) % span
+        p = 0.05 + 0.9 * (synth_u64(seed, 0xB, 3 * i + 2) >> 11) / float(1 << 53)
+        out.append((h, w, p))
+    return out
 
 
 def header_functions(path: str = HEADER_PATH):

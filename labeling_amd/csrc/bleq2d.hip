@@ -1,5 +1,5 @@
 // bleq2d.hip -- 2D block-equivalence labeling (2x2 blocks, 8-connectivity) for
-// gfx950.
+// gfx950: single images and batches of images of uneven sizes.
 //
 // Same problem as the reference's TOCLLabelEquivalenceX2 (LabelingAlgs.cpp:948-1084,
 // kernels LBEQ2_* LabelingAlgs.cl:133-319): pixels are grouped into 2x2 blocks,
@@ -10,33 +10,33 @@
 // per pass; here the equivalences are resolved by union-find in four launches
 // and no host synchronisation:
 //
-//   1. tile    one workgroup per 128x16-block tile (256x32 px): 16-byte row
-//              loads -> 4-bit block masks; union-find of the tile's blocks in
-//              LDS; writes one packed word per block, parent << 4 | mask, whose
+//   1. tile    one workgroup per TBW x TBH-block tile: 16-byte row loads ->
+//              4-bit block masks; union-find of the tile's blocks in LDS;
+//              writes one packed word per block, parent << 4 | mask, whose
 //              parent is the tile-local root (global block index).
 //   2. merge   one thread per block on a tile border: unions with each
 //              8-neighbour block in another tile that it touches (atomicMin
-//              links in the global forest).
-//   3. flatten one thread per tile-border block: points the block's tile-local
-//              root straight at its global root.
+//              links between tile-local roots).
+//   3. flatten one thread per tile-border block: the block and its tile-local
+//              root are pointed straight at the global root.
 //   4. final   one thread per 4 blocks: root of the block (<= 2 hops), writes
 //              the 8 labels of the 2x2 pixels of each block, 0 for background,
 //              and counts roots.
 //
+// Every kernel reads its image geometry from a "source": One (a single image
+// passed by value) or Many (a device table of images; a workgroup finds its
+// image by binary search over the images' first tile / workgroup ids), so a
+// batch of thousands of images runs as the same four launches.
+//
 // HBM traffic per pixel: 1 B image + 1 B block words in (1), 1 B block words +
 // 4 B labels in (4), plus the border blocks in (2)-(3).
-#include "lbl_internal.hpp"
+#include "bleq2d.hpp"
 #include "uf_device.hpp"
 
 namespace lbl {
 namespace {
 
-constexpr uint32_t TBW = 128;            // tile width in blocks
-constexpr uint32_t TBH = 16;             // tile height in blocks
 constexpr uint32_t SEG = 8;              // blocks per thread in the tile kernel
-constexpr uint32_t SEGS = TBW / SEG;     // threads per block row
-constexpr uint32_t NT_TILE = SEGS * TBH; // 256
-constexpr uint32_t JOBS = 2 * TBW + 2 * (TBH - 2);  // border blocks per tile
 constexpr uint32_t FINAL_BLOCKS = 4;     // blocks per thread in the final kernel
 constexpr uint32_t NT_FINAL = 256;
 
@@ -51,31 +51,78 @@ __device__ __forceinline__ bool adj_upleft(uint32_t c, uint32_t n) { return (c &
 __device__ __forceinline__ bool adj_up(uint32_t c, uint32_t n) { return (c & 0x3u) && (n & 0xCu); }
 __device__ __forceinline__ bool adj_upright(uint32_t c, uint32_t n) { return (c & 0x2u) && (n & 0x4u); }
 
-// ---- 1. tile kernel ---------------------------------------------------------
+// ---- geometry sources ----------------------------------------------------------
 
-template <bool VEC>
-__global__ __launch_bounds__(NT_TILE) void bleq2d_tile(const uint8_t *__restrict__ px, size_t pitch,
-                                                        uint32_t h, uint32_t w, uint32_t bw, uint32_t bh,
-                                                        uint32_t tiles_x, uint32_t *__restrict__ P,
-                                                        uint32_t *count)
+struct One {
+    Geom2D g;
+    __device__ __forceinline__ const Geom2D &by_tile(uint32_t gt, uint32_t &lt) const
+    {
+        lt = gt;
+        return g;
+    }
+    __device__ __forceinline__ const Geom2D &by_final(uint32_t gw, uint32_t &lw) const
+    {
+        lw = gw;
+        return g;
+    }
+};
+
+struct Many {
+    const Geom2D *g;
+    uint32_t n;
+    // last image whose first id is <= id
+    template <int FIELD>
+    __device__ __forceinline__ uint32_t find(uint32_t id) const
+    {
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            const uint32_t base = FIELD == 0 ? g[mid].tile_base : g[mid].final_base;
+            if (base <= id) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    }
+    __device__ __forceinline__ const Geom2D &by_tile(uint32_t gt, uint32_t &lt) const
+    {
+        const Geom2D &r = g[find<0>(gt)];
+        lt = gt - r.tile_base;
+        return r;
+    }
+    __device__ __forceinline__ const Geom2D &by_final(uint32_t gw, uint32_t &lw) const
+    {
+        const Geom2D &r = g[find<1>(gw)];
+        lw = gw - r.final_base;
+        return r;
+    }
+};
+
+// ---- 1. tile kernel -------------------------------------------------------------
+
+template <uint32_t TBW, uint32_t TBH, class Src>
+__global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
 {
+    constexpr uint32_t SEGS = TBW / SEG;
     __shared__ uint32_t par[TBH * TBW];
     __shared__ uint32_t mw[TBH][SEGS + 2];   // 8 nibbles per thread, guard column on each side
 
+    uint32_t lt;
+    const Geom2D &G = src.by_tile(blockIdx.x, lt);
+    const uint32_t h = G.h, w = G.w, bw = G.bw, bh = G.bh;
     const uint32_t t = threadIdx.x, r = t / SEGS, s = t % SEGS;
-    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const uint32_t tx = lt % G.tiles_x, ty = lt / G.tiles_x;
     const uint32_t bx0 = tx * TBW, by0 = ty * TBH;
     const uint32_t by = by0 + r, bxs = bx0 + s * SEG;
-    if (blockIdx.x == 0 && t == 0 && count) *count = 0;
+    if (lt == 0 && t == 0 && G.count) *G.count = 0;
 
     // masks of this thread's 8 blocks
     uint32_t m8 = 0;
     if (by < bh && bxs < bw) {
         const uint32_t y0 = 2 * by, x0 = 2 * bxs;
-        const uint8_t *row0 = px + (size_t)y0 * pitch;
-        const uint8_t *row1 = row0 + pitch;
+        const uint8_t *row0 = G.px + (size_t)y0 * G.pitch;
+        const uint8_t *row1 = row0 + G.pitch;
         const bool two_rows = y0 + 1 < h;
-        if (VEC && two_rows && x0 + 16 <= w) {
+        if ((G.flags & kVecIn) && two_rows && x0 + 16 <= w) {
             const uint4 a = *reinterpret_cast<const uint4 *>(row0 + x0);
             const uint4 b = *reinterpret_cast<const uint4 *>(row1 + x0);
             m8 = quad_masks(a.x, b.x) | (quad_masks(a.y, b.y) << 8) | (quad_masks(a.z, b.z) << 16) |
@@ -168,7 +215,7 @@ __global__ __launch_bounds__(NT_TILE) void bleq2d_tile(const uint8_t *__restrict
             const uint32_t g = (by0 + root / TBW) * bw + bx0 + root % TBW;
             out[k] = (g << 4) | m;
         }
-        uint32_t *dst = P + (size_t)by * bw + bxs;
+        uint32_t *dst = G.P + (size_t)by * bw + bxs;
         if ((bw % 4u) == 0 && bxs + SEG <= bw) {
             reinterpret_cast<uint4 *>(dst)[0] = make_uint4(out[0], out[1], out[2], out[3]);
             reinterpret_cast<uint4 *>(dst)[1] = make_uint4(out[4], out[5], out[6], out[7]);
@@ -179,39 +226,48 @@ __global__ __launch_bounds__(NT_TILE) void bleq2d_tile(const uint8_t *__restrict
     }
 }
 
-// ---- 2./3. tile-border kernels -----------------------------------------------
+// ---- 2./3. tile-border kernels -----------------------------------------------------
 
-// j-th border block of a tile: top row, bottom row, then the left and right
-// columns without their corner blocks.
-__device__ __forceinline__ void border_block(uint32_t j, uint32_t bx0, uint32_t by0, uint32_t &bx, uint32_t &by)
-{
-    if (j < TBW) {
-        bx = bx0 + j;
-        by = by0;
-    } else if (j < 2 * TBW) {
-        bx = bx0 + (j - TBW);
-        by = by0 + TBH - 1;
-    } else if (j < 2 * TBW + (TBH - 2)) {
-        bx = bx0;
-        by = by0 + 1 + (j - 2 * TBW);
-    } else {
-        bx = bx0 + TBW - 1;
-        by = by0 + 1 + (j - 2 * TBW - (TBH - 2));
+template <uint32_t TBW, uint32_t TBH>
+struct Border {
+    static constexpr uint32_t JOBS = 2 * TBW + 2 * (TBH - 2);
+
+    // j-th border block of a tile: top row, bottom row, then the left and right
+    // columns without their corner blocks.
+    __device__ static void block(uint32_t j, uint32_t bx0, uint32_t by0, uint32_t &bx, uint32_t &by)
+    {
+        if (j < TBW) {
+            bx = bx0 + j;
+            by = by0;
+        } else if (j < 2 * TBW) {
+            bx = bx0 + (j - TBW);
+            by = by0 + TBH - 1;
+        } else if (j < 2 * TBW + (TBH - 2)) {
+            bx = bx0;
+            by = by0 + 1 + (j - 2 * TBW);
+        } else {
+            bx = bx0 + TBW - 1;
+            by = by0 + 1 + (j - 2 * TBW - (TBH - 2));
+        }
     }
-}
+};
 
-__global__ __launch_bounds__(256) void bleq2d_merge(uint32_t *P, uint32_t bw, uint32_t bh, uint32_t tiles_x,
-                                                     uint32_t ntiles)
+template <uint32_t TBW, uint32_t TBH, class Src>
+__global__ __launch_bounds__(256) void bleq2d_merge(Src src, uint32_t total_tiles)
 {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t tile = g / JOBS, j = g % JOBS;
-    if (tile >= ntiles) return;
-    const uint32_t bx0 = (tile % tiles_x) * TBW, by0 = (tile / tiles_x) * TBH;
+    using B = Border<TBW, TBH>;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gt = gid / B::JOBS, j = gid % B::JOBS;
+    if (gt >= total_tiles) return;
+    uint32_t lt;
+    const Geom2D &G = src.by_tile(gt, lt);
+    const uint32_t bw = G.bw, bh = G.bh;
+    const uint32_t bx0 = (lt % G.tiles_x) * TBW, by0 = (lt / G.tiles_x) * TBH;
     uint32_t bx, by;
-    border_block(j, bx0, by0, bx, by);
+    B::block(j, bx0, by0, bx, by);
     if (bx >= bw || by >= bh) return;
-    const uint32_t b = by * bw + bx;
-    const uint32_t vb = P[b];
+    uint32_t *P = G.P;
+    const uint32_t vb = P[by * bw + bx];
     const uint32_t m = vb & 15u;
     if (!m) return;
     // earlier neighbours: left, up-left, up, up-right
@@ -221,58 +277,76 @@ __global__ __launch_bounds__(256) void bleq2d_merge(uint32_t *P, uint32_t bw, ui
         const uint32_t nx = bx + ndx[k], ny = by + ndy[k];   // unsigned wrap -> out of range
         if (nx >= bw || ny >= bh) continue;
         if (nx - bx0 < TBW && ny - by0 < TBH) continue;    // same tile: done in the tile kernel
-        const uint32_t n = ny * bw + nx;
-        const uint32_t vn = P[n];
+        const uint32_t vn = P[ny * bw + nx];
         const uint32_t nm = vn & 15u;
         const bool adj = k == 0 ? adj_left(m, nm) : (k == 1 ? adj_upleft(m, nm) : (k == 2 ? adj_up(m, nm) : adj_upright(m, nm)));
         if (adj) Forest::unite(P, vb >> 4, vn >> 4);   // tile-local roots
     }
 }
 
-__global__ __launch_bounds__(256) void bleq2d_flatten(uint32_t *P, uint32_t bw, uint32_t bh, uint32_t tiles_x,
-                                                       uint32_t ntiles)
+template <uint32_t TBW, uint32_t TBH, class Src>
+__global__ __launch_bounds__(256) void bleq2d_flatten(Src src, uint32_t total_tiles)
 {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t tile = g / JOBS, j = g % JOBS;
-    if (tile >= ntiles) return;
-    const uint32_t bx0 = (tile % tiles_x) * TBW, by0 = (tile / tiles_x) * TBH;
+    using B = Border<TBW, TBH>;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gt = gid / B::JOBS, j = gid % B::JOBS;
+    if (gt >= total_tiles) return;
+    uint32_t lt;
+    const Geom2D &G = src.by_tile(gt, lt);
+    const uint32_t bx0 = (lt % G.tiles_x) * TBW, by0 = (lt / G.tiles_x) * TBH;
     uint32_t bx, by;
-    border_block(j, bx0, by0, bx, by);
-    if (bx >= bw || by >= bh) return;
-    const uint32_t b = by * bw + bx;
-    if (!(P[b] & 15u)) return;
-    Forest::compress(P, b);
+    B::block(j, bx0, by0, bx, by);
+    if (bx >= G.bw || by >= G.bh) return;
+    const uint32_t b = by * G.bw + bx;
+    if (G.P[b] & 15u) Forest::compress(G.P, b);
 }
 
-// ---- 4. final kernel -----------------------------------------------------------
+// ---- 4. final kernel -------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t label_of(uint32_t root, uint32_t mode, uint32_t bw, uint32_t ref_bw)
+// Eight labels of one pixel row starting at column x0 (x0 % 8 == 0): the widest
+// stores the row's actual address alignment allows (16, 8 or 4 bytes), so
+// images of any width keep vector stores on most rows.
+__device__ __forceinline__ void store_row8(uint32_t *dst, const uint32_t v[8], uint32_t x0, uint32_t w)
 {
-    if (mode == LBL_LABELS_REFERENCE) return 1u + (root % bw) + (root / bw) * ref_bw;
-    return root + 1u;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(dst);
+    if (x0 + 8 <= w) {
+        if ((a & 15u) == 0) {
+            store_nt4(dst, v[0], v[1], v[2], v[3]);
+            store_nt4(dst + 4, v[4], v[5], v[6], v[7]);
+            return;
+        }
+        if ((a & 7u) == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) store_nt2(dst + i, v[i], v[i + 1]);
+            return;
+        }
+    }
+    for (uint32_t i = 0; i < 8 && x0 + i < w; ++i) dst[i] = v[i];
 }
 
-template <bool VEC>
-__global__ __launch_bounds__(NT_FINAL) void bleq2d_final(const uint32_t *__restrict__ P, uint32_t *__restrict__ labels,
-                                                          uint32_t h, uint32_t w, uint32_t bw, uint32_t bh,
-                                                          uint32_t xgroups, uint32_t *count, uint32_t mode,
-                                                          uint32_t ref_bw)
+template <class Src>
+__global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
 {
-    const uint32_t by = blockIdx.x / xgroups;
-    const uint32_t bxs = ((blockIdx.x % xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    uint32_t lw;
+    const Geom2D &G = src.by_final(blockIdx.x, lw);
+    const uint32_t h = G.h, w = G.w, bw = G.bw, bh = G.bh;
+    const uint32_t by = lw / G.xgroups;
+    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    const uint32_t *__restrict__ P = G.P;
     uint32_t roots = 0;
     if (by < bh && bxs < bw) {
-        const uint32_t *src = P + (size_t)by * bw + bxs;
+        const uint32_t *src_w = P + (size_t)by * bw + bxs;
         uint32_t v[FINAL_BLOCKS];
-        if (VEC && bxs + FINAL_BLOCKS <= bw) {
-            const uint4 q = *reinterpret_cast<const uint4 *>(src);
+        if ((reinterpret_cast<uintptr_t>(src_w) & 15u) == 0 && bxs + FINAL_BLOCKS <= bw) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(src_w);
             v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
         } else {
-            for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = bxs + k < bw ? src[k] : 0u;
+            for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = bxs + k < bw ? src_w[k] : 0u;
         }
         uint32_t lab[FINAL_BLOCKS];
         uint32_t prev_parent = 0xFFFFFFFFu, prev_root = 0;
         const uint32_t b0 = by * bw + bxs;
+        const bool ref = (G.flags & kRefLabels) != 0;
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
             lab[k] = 0;
@@ -282,10 +356,11 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(const uint32_t *__restr
             prev_parent = p;
             prev_root = root;
             roots += root == b0 + k;
-            lab[k] = label_of(root, mode, bw, ref_bw);
+            // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
+            lab[k] = ref ? 1u + (root % bw) + (root / bw) * G.ref_bw : root + 1u;
         }
         const uint32_t y0 = 2 * by, x0 = 2 * bxs;
-        uint32_t *row0 = labels + (size_t)y0 * w + x0;
+        uint32_t *row0 = G.labels + (size_t)y0 * w + x0;
         uint32_t *row1 = row0 + w;
         const bool two_rows = y0 + 1 < h;
         uint32_t top[2 * FINAL_BLOCKS], bot[2 * FINAL_BLOCKS];
@@ -297,23 +372,68 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(const uint32_t *__restr
             bot[2 * k] = (m & 4u) ? lab[k] : 0u;
             bot[2 * k + 1] = (m & 8u) ? lab[k] : 0u;
         }
-        if (VEC && two_rows && x0 + 2 * FINAL_BLOCKS <= w) {
-            store_nt4(row0, top[0], top[1], top[2], top[3]);
-            store_nt4(row0 + 4, top[4], top[5], top[6], top[7]);
-            store_nt4(row1, bot[0], bot[1], bot[2], bot[3]);
-            store_nt4(row1 + 4, bot[4], bot[5], bot[6], bot[7]);
-        } else {
-            for (uint32_t i = 0; i < 2 * FINAL_BLOCKS; ++i) {
-                if (x0 + i >= w) break;
-                row0[i] = top[i];
-                if (two_rows) row1[i] = bot[i];
-            }
-        }
+        store_row8(row0, top, x0, w);
+        if (two_rows) store_row8(row1, bot, x0, w);
     }
-    if (count) wave_add_atomic(count, roots);
+    if (G.count) wave_add_atomic(G.count, roots);   // one image per workgroup: uniform branch
+}
+
+// ---- host side --------------------------------------------------------------------
+
+template <uint32_t TBW, uint32_t TBH>
+struct Variant {
+    static constexpr uint32_t THREADS = (TBW / SEG) * TBH;
+
+    template <class Src>
+    static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
+                             KernelTimer *kt, bool flatten)
+    {
+        const uint32_t border_grid = (uint32_t)(((uint64_t)total_tiles * Border<TBW, TBH>::JOBS + 255) / 256);
+        hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src>), dim3(total_tiles), dim3(THREADS), 0, s, src);
+        if (kt) kt->mark(s, "bleq2d_tile");
+        hipLaunchKernelGGL((bleq2d_merge<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
+        if (kt) kt->mark(s, "bleq2d_merge");
+        if (flatten) {
+            hipLaunchKernelGGL((bleq2d_flatten<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
+            if (kt) kt->mark(s, "bleq2d_flatten");
+        }
+        hipLaunchKernelGGL((bleq2d_final<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+        if (kt) kt->mark(s, "bleq2d_final");
+        return hipGetLastError();
+    }
+};
+
+template <class Src>
+hipError_t dispatch(uint32_t tile_id, const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
+                    KernelTimer *kt, bool flatten)
+{
+    switch (tile_id) {
+    case 1: return Variant<64, 16>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 2: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 3: return Variant<256, 8>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 4: return Variant<64, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
+    default: return Variant<128, 16>::launch(src, total_tiles, total_final, s, kt, flatten);
+    }
 }
 
 }  // namespace
+
+const TileShape2D kBleq2dTiles[kBleq2dTileCount] = {{128, 16}, {64, 16}, {128, 32}, {256, 8}, {64, 32}};
+
+void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
+{
+    const TileShape2D ts = kBleq2dTiles[tile_id < kBleq2dTileCount ? tile_id : 0];
+    g.bw = (g.w + 1u) / 2;
+    g.bh = (g.h + 1u) / 2;
+    g.tiles_x = (g.bw + ts.bw - 1) / ts.bw;
+    g.ntiles = g.tiles_x * ((g.bh + ts.bh - 1) / ts.bh);
+    g.xgroups = (g.bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
+    g.nfinal = g.xgroups * g.bh;
+    g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
+    g.flags &= kRefLabels;
+    if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
+    if (g.w % 8 == 0 && reinterpret_cast<uintptr_t>(g.labels) % 16 == 0) g.flags |= kVecOut;
+}
 
 size_t bleq2d_workspace(uint32_t h, uint32_t w)
 {
@@ -323,37 +443,28 @@ size_t bleq2d_workspace(uint32_t h, uint32_t w)
 
 hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
 {
-    const uint32_t bw = (im.w + 1u) / 2, bh = (im.h + 1u) / 2;
-    const uint32_t tiles_x = (bw + TBW - 1) / TBW, tiles_y = (bh + TBH - 1) / TBH;
-    const uint32_t ntiles = tiles_x * tiles_y;
-    uint32_t *P = static_cast<uint32_t *>(ws);
-    const bool vec_in = (im.pitch % 16 == 0) && (reinterpret_cast<uintptr_t>(im.px) % 16 == 0);
-    const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
-    const uint32_t ref_bw = (uint32_t)((((uint64_t)im.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
-    const uint32_t border_grid = (uint32_t)(((uint64_t)ntiles * JOBS + 255) / 256);
-    const uint32_t xgroups = (bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
-
+    One one;
+    Geom2D &g = one.g;
+    g = Geom2D{};
+    g.px = im.px;
+    g.pitch = im.pitch;
+    g.h = im.h;
+    g.w = im.w;
+    g.labels = im.labels;
+    g.count = im.count;
+    g.P = static_cast<uint32_t *>(ws);
+    g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
+    bleq2d_geometry(g, opt.tile2d);
     if (kt) kt->begin(s);
-    if (vec_in)
-        hipLaunchKernelGGL(bleq2d_tile<true>, dim3(ntiles), dim3(NT_TILE), 0, s, im.px, im.pitch, im.h, im.w, bw,
-                           bh, tiles_x, P, im.count);
-    else
-        hipLaunchKernelGGL(bleq2d_tile<false>, dim3(ntiles), dim3(NT_TILE), 0, s, im.px, im.pitch, im.h, im.w, bw,
-                           bh, tiles_x, P, im.count);
-    if (kt) kt->mark(s, "bleq2d_tile");
-    hipLaunchKernelGGL(bleq2d_merge, dim3(border_grid), dim3(256), 0, s, P, bw, bh, tiles_x, ntiles);
-    if (kt) kt->mark(s, "bleq2d_merge");
-    hipLaunchKernelGGL(bleq2d_flatten, dim3(border_grid), dim3(256), 0, s, P, bw, bh, tiles_x, ntiles);
-    if (kt) kt->mark(s, "bleq2d_flatten");
-    const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh));
-    if (vec_out)
-        hipLaunchKernelGGL(bleq2d_final<true>, fgrid, dim3(NT_FINAL), 0, s, P, im.labels, im.h, im.w, bw, bh,
-                           xgroups, im.count, (uint32_t)opt.mode, ref_bw);
-    else
-        hipLaunchKernelGGL(bleq2d_final<false>, fgrid, dim3(NT_FINAL), 0, s, P, im.labels, im.h, im.w, bw, bh,
-                           xgroups, im.count, (uint32_t)opt.mode, ref_bw);
-    if (kt) kt->mark(s, "bleq2d_final");
-    return hipGetLastError();
+    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten);
+}
+
+hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,
+                            uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt)
+{
+    Many many{d_geoms, n};
+    if (kt) kt->begin(s);
+    return dispatch(tile_id, many, total_tiles, total_final, s, kt, flatten);
 }
 
 }  // namespace lbl

@@ -1,0 +1,45 @@
+// bleq2d.hpp -- launch geometry of the 2D block-equivalence kernels, shared by
+// the single-image entry points and the batch planner.
+#pragma once
+
+#include "lbl_internal.hpp"
+
+namespace lbl {
+
+enum : uint32_t {
+    kVecIn = 1u,       // image rows start 16-byte aligned (pitch and base)
+    kVecOut = 2u,      // label rows start 16-byte aligned (w % 8 == 0, base)
+    kRefLabels = 4u,   // write the reference's raw label values
+};
+
+// One image as the kernels see it (also the element of a batch's device table).
+struct Geom2D {
+    const uint8_t *px;
+    uint32_t *labels;
+    uint32_t *count;       // component count (may be null)
+    uint32_t *P;           // block words: bw * bh uint32
+    uint64_t pitch;
+    uint32_t h, w, bw, bh;
+    uint32_t tiles_x, ntiles;
+    uint32_t xgroups, nfinal;   // final-kernel workgroups: xgroups per block row, nfinal total
+    uint32_t tile_base;         // first global tile id of this image (batch)
+    uint32_t final_base;        // first global final workgroup id (batch)
+    uint32_t flags;
+    uint32_t ref_bw;            // block-grid width of the reference's padded image
+};
+
+struct TileShape2D {
+    uint32_t bw, bh;   // tile size in blocks
+};
+
+constexpr uint32_t kBleq2dTileCount = 5;
+extern const TileShape2D kBleq2dTiles[kBleq2dTileCount];
+
+// Fill bw..nfinal and the alignment flags for a tile shape (px, labels, pitch,
+// h, w and the kRefLabels flag must be set).
+void bleq2d_geometry(Geom2D &g, uint32_t tile_id);
+
+hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,
+                            uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt);
+
+}  // namespace lbl

@@ -5,6 +5,7 @@
 // padding, per-call allocation and blocking per-iteration transfers: device
 // buffers are cached on the context, the labeling kernels are enqueued
 // back-to-back on one stream, and errors are returned as status codes.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -294,8 +295,13 @@ static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, 
     lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w));
     if (st != LBL_OK) return st;
     KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
-    hipError_t e = alg == LBL_BLEQ ? bleq2d_run(im, ctx->opt, ctx->ws, s, kt)
-                                   : lbeq2d_run(im, conn, ctx->opt, ctx->ws, s, kt);
+    Options opt = ctx->opt;
+    auto it = ctx->tuning.find(TuneKey{(int)alg, 0, size_class_of((uint64_t)im.h * im.w)});
+    if (it != ctx->tuning.end()) {   // autotuned launch configuration for this size class
+        opt.tile2d = it->second.tile;
+        opt.flatten = it->second.flatten;
+    }
+    hipError_t e = alg == LBL_BLEQ ? bleq2d_run(im, opt, ctx->ws, s, kt) : lbeq2d_run(im, conn, opt, ctx->ws, s, kt);
     if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq2d launch" : "lbeq2d launch", e);
     return LBL_OK;
 }
@@ -412,4 +418,32 @@ extern "C" lbl_status lbl_label3d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *vox,
     Image3D im{ctx->d_in, d, h, w, ctx->d_lab, ctx->d_misc};
     if ((st = run3d(ctx, alg, im, s)) != LBL_OK) return st;
     return finish_host(ctx, labels, n, n_components, elapsed_us);
+}
+
+extern "C" lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_image2d *images, uint32_t n, int conn,
+                                       uint32_t *d_ncomp, void *stream)
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!images || n == 0) return fail(ctx, LBL_EINVAL, "lbl_label2d_each: empty image list");
+    int c = 0;
+    lbl_status st;
+    if ((st = normalize_conn2d(ctx, alg, conn, &c)) != LBL_OK) return st;
+    if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
+    size_t need = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const lbl_image2d &im = images[i];
+        if (!im.px || !im.labels) return fail(ctx, LBL_EINVAL, "lbl_label2d_each: null buffer");
+        if ((st = check_size2d(ctx, alg, im.h, im.w, im.pitch_bytes)) != LBL_OK) return st;
+        need = std::max(need, workspace2d(alg, im.h, im.w));
+    }
+    DeviceGuard g(ctx->device);
+    if ((st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, need)) != LBL_OK) return st;
+    hipStream_t s = pick_stream(ctx, stream);
+    for (uint32_t i = 0; i < n; ++i) {
+        const lbl_image2d &im = images[i];
+        Image2D one{im.px, im.pitch_bytes, im.h, im.w, im.labels, d_ncomp ? d_ncomp + i : nullptr};
+        if ((st = run2d(ctx, alg, c, one, s)) != LBL_OK) return st;   // the workspace is reused in stream order
+    }
+    return LBL_OK;
 }

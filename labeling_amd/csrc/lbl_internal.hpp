@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -34,7 +35,25 @@ struct Options {
     lbl_label_mode mode = LBL_LABELS_ROOT;
     uint32_t iter_cap = 0;       // lbeq tile sweeps cap, 0 = none
     uint32_t *status = nullptr;  // device word; kernels OR in error bits (bit0 = iteration cap hit)
+    uint32_t tile2d = 0;         // bleq 2D tile shape id (kBleq2dTiles)
+    bool flatten = true;         // run the flatten pass (else the final pass chases full paths)
 };
+
+// Autotuned launch configuration for a workload class.
+struct TuneKey {
+    int alg, kind;        // kind: 0 = single 2D image, 1 = 2D batch
+    int size_class;       // floor(log2(pixels per image))
+    bool operator<(const TuneKey &o) const
+    {
+        return alg != o.alg ? alg < o.alg : (kind != o.kind ? kind < o.kind : size_class < o.size_class);
+    }
+};
+struct TuneVal {
+    uint32_t tile;
+    bool flatten;
+    float us;
+};
+int size_class_of(uint64_t pixels);
 
 // A named kernel launch recorded for optional per-kernel timing.
 struct KernelTimer {
@@ -87,4 +106,5 @@ struct lbl_ctx {
     uint32_t *d_misc = nullptr;   // [0] = component count, [1] = status bits
     uint32_t *h_misc = nullptr;   // pinned host mirror of d_misc
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<lbl::TuneKey, lbl::TuneVal> tuning;
 };

@@ -198,6 +198,14 @@ __device__ __forceinline__ void store_nt4(uint32_t *p, uint32_t a, uint32_t b, u
     __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(p));
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void store_nt2(uint32_t *p, uint32_t a, uint32_t b)
+{
+    u32x2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x2 *>(p));
+}
+
 // ---- wave helpers ----------------------------------------------------------
 
 // Add `v` over the workgroup's waves into *dst with one atomic per wave.

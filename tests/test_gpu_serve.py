@@ -1,0 +1,91 @@
+"""Continuous-batching server: every request's labels and component count in a
+mixed, overlapping stream must equal the single-request result on the same
+input (tolerance 0: labels are integers)."""
+import numpy as np
+import pytest
+
+import labeling_amd as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_requests(n, seed):
+    reqs = []
+    for i in range(n):
+        u = L.synth_u64(seed, 0x5E, i)
+        kind = u % 10
+        if kind < 8:   # 2D
+            h = 1 + L.synth_u64(seed, 0x5F, 2 * i) % 700
+            w = 1 + L.synth_u64(seed, 0x5F, 2 * i + 1) % 700
+            img = L.bernoulli((h, w), 0.1 + 0.8 * (u % 97) / 97, seed, stream=i)
+            alg, conn = ("lbeq", 4 if kind == 7 else 8) if kind >= 6 else ("bleq", 0)
+        else:          # 3D
+            s = 2 + L.synth_u64(seed, 0x60, i) % 60
+            img = L.bernoulli((s, s + 1, s + 3), 0.2 + 0.1 * (u % 3), seed, stream=i)
+            alg, conn = ("bleq" if kind == 8 else "lbeq"), 0
+        reqs.append((img, alg, conn))
+    return reqs
+
+
+def _run(labeler, reqs, **cfg):
+    srv = L.Server(labeler, **cfg)
+    pins = []
+    tickets = {}
+    try:
+        for k, (img, alg, conn) in enumerate(reqs):
+            pin_in = L.PinnedArray(img.shape, np.uint8)
+            pin_in.array[...] = img
+            pin_out = L.PinnedArray(img.shape, np.uint32)
+            pins.append((pin_in, pin_out))
+            t = srv.submit(pin_in.ptr, pin_out.ptr, img.shape, alg, conn)
+            tickets[t] = k
+        srv.drain()
+        done = srv.poll(cap=len(reqs) + 8)
+        assert len(done) == len(reqs)
+        results = {}
+        for ticket, status, ncomp, t_sub, t_start, t_done in done:
+            assert status == L.OK
+            assert t_sub <= t_start <= t_done
+            k = tickets[ticket]
+            results[k] = (pins[k][1].array.copy(), ncomp)
+        return results
+    finally:
+        srv.close()
+        for a, b in pins:
+            a.free()
+            b.free()
+
+
+@pytest.mark.parametrize("cfg", [dict(streams=4, hbm_budget_bytes=256 << 20, max_batch=8),
+                                 dict(streams=2, hbm_budget_bytes=12 << 20, max_batch=3),
+                                 dict(streams=1, hbm_budget_bytes=64 << 20, max_batch=1)],
+                         ids=["wide", "tight-memory", "serial"])
+def test_mix_equals_alone(labeler, cfg):
+    reqs = _mixed_requests(60, seed=4)
+    print(f"[serve {cfg}]", flush=True)
+    got = _run(labeler, reqs, **cfg)
+    for k, (img, alg, conn) in enumerate(reqs):
+        if img.ndim == 2:
+            alone, n, _ = labeler.label2d(img, alg, conn=conn)
+        else:
+            alone, n, _ = labeler.label3d(img, alg)
+        lab, ncomp = got[k]
+        np.testing.assert_array_equal(lab, alone, err_msg=f"request {k} {alg} {img.shape}")
+        assert ncomp == n
+
+
+def test_admission_limits(labeler):
+    srv = L.Server(labeler, streams=2, hbm_budget_bytes=2 << 20, max_batch=4)
+    try:
+        big = L.PinnedArray((1024, 1024), np.uint8)
+        out = L.PinnedArray((1024, 1024), np.uint32)
+        with pytest.raises(L.LabelingError) as e:   # 1 MiB in + 4 MiB labels > 2 MiB arena
+            srv.submit(big.ptr, out.ptr, (1024, 1024), "bleq")
+        assert e.value.status == L.ENOMEM
+        with pytest.raises(L.LabelingError) as e:
+            srv.submit(big.ptr, out.ptr, (16, 16), "bleq", conn=4)
+        assert e.value.status == L.EUNSUPPORTED
+        big.free()
+        out.free()
+    finally:
+        srv.close()
