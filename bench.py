@@ -351,6 +351,9 @@ def main():
     import torch   # initialise torch's HIP runtime before the library binds to it
 
     torch.zeros(1, device="cuda:0")
+    # a real (non-null) stream: the library's kernels and torch's timing events
+    # must go to the same stream, and a null handle would select the library's own
+    torch.cuda.set_stream(torch.cuda.Stream())
     if a.calib_copy:
         # plain device copy of a known size: the reference point that scales the
         # FETCH_SIZE / WRITE_SIZE counters into HBM bytes

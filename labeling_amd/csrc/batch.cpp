@@ -20,6 +20,7 @@
 #include <sstream>
 
 #include "bleq2d.hpp"
+#include "uf_device.hpp"
 
 using namespace lbl;
 
@@ -108,10 +109,11 @@ size_t plan(lbl_batch *b, uint32_t tile)
         bleq2d_geometry(g, tile);
         g.tile_base = tb;
         g.final_base = fb;
-        g.P = reinterpret_cast<uint32_t *>(off);   // offset, rebased once ws exists
+        g.P = reinterpret_cast<uint32_t *>(off);   // offsets, rebased once ws exists
+        g.slots = reinterpret_cast<uint32_t *>(off + align256((uint64_t)g.bw * g.bh * sizeof(uint32_t)));
         tb += g.ntiles;
         fb += g.nfinal;
-        off += align256((uint64_t)g.bw * g.bh * sizeof(uint32_t));
+        off += align256((uint64_t)g.bw * g.bh * sizeof(uint32_t)) + kCountBytes;
         b->geoms[i] = g;
     }
     b->total_tiles = tb;
@@ -124,7 +126,10 @@ lbl_status upload(lbl_batch *b)
 {
     lbl_ctx *ctx = b->ctx;
     std::vector<Geom2D> dev = b->geoms;
-    for (auto &g : dev) g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(b->ws) + reinterpret_cast<uintptr_t>(g.P));
+    for (auto &g : dev) {
+        g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(b->ws) + reinterpret_cast<uintptr_t>(g.P));
+        g.slots = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(b->ws) + reinterpret_cast<uintptr_t>(g.slots));
+    }
     B_HIP(ctx, hipMemcpyAsync(b->d_geoms, dev.data(), dev.size() * sizeof(Geom2D), hipMemcpyHostToDevice, ctx->stream));
     B_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return LBL_OK;

@@ -65,6 +65,12 @@ struct One {
         lw = gw;
         return g;
     }
+    __device__ __forceinline__ const Geom2D &by_image(uint32_t i, uint32_t &li) const
+    {
+        li = i;
+        return g;
+    }
+    uint32_t images() const { return g.count ? 1u : 0u; }
 };
 
 struct Many {
@@ -95,6 +101,12 @@ struct Many {
         lw = gw - r.final_base;
         return r;
     }
+    __device__ __forceinline__ const Geom2D &by_image(uint32_t i, uint32_t &li) const
+    {
+        li = i;
+        return g[i];
+    }
+    uint32_t images() const { return n; }
 };
 
 // ---- 1. tile kernel -------------------------------------------------------------
@@ -113,7 +125,7 @@ __global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
     const uint32_t tx = lt % G.tiles_x, ty = lt / G.tiles_x;
     const uint32_t bx0 = tx * TBW, by0 = ty * TBH;
     const uint32_t by = by0 + r, bxs = bx0 + s * SEG;
-    if (lt == 0 && t == 0 && G.count) *G.count = 0;
+    if (lt == 0 && G.count) count_clear(G.slots);
 
     // masks of this thread's 8 blocks
     uint32_t m8 = 0;
@@ -252,53 +264,84 @@ struct Border {
     }
 };
 
+// Border job of thread gid: its image, block and packed word (valid = the job
+// exists and the block is foreground).
+template <uint32_t TBW, uint32_t TBH, class Src>
+__device__ __forceinline__ bool border_job(const Src &src, uint32_t gid, uint32_t total_tiles, const Geom2D *&G,
+                                           uint32_t &bx, uint32_t &by, uint32_t &bx0, uint32_t &by0, uint32_t &vb)
+{
+    using B = Border<TBW, TBH>;
+    const uint32_t gt = gid / B::JOBS, j = gid % B::JOBS;
+    if (gt >= total_tiles) return false;
+    uint32_t lt;
+    G = &src.by_tile(gt, lt);
+    bx0 = (lt % G->tiles_x) * TBW;
+    by0 = (lt / G->tiles_x) * TBH;
+    B::block(j, bx0, by0, bx, by);
+    if (bx >= G->bw || by >= G->bh) return false;
+    vb = G->P[by * G->bw + bx];
+    return (vb & 15u) != 0;
+}
+
+// Consecutive border blocks of a tile usually repeat the same union (same
+// tile-local roots on both sides); only the first lane of such a run unites.
 template <uint32_t TBW, uint32_t TBH, class Src>
 __global__ __launch_bounds__(256) void bleq2d_merge(Src src, uint32_t total_tiles)
 {
-    using B = Border<TBW, TBH>;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t gt = gid / B::JOBS, j = gid % B::JOBS;
-    if (gt >= total_tiles) return;
-    uint32_t lt;
-    const Geom2D &G = src.by_tile(gt, lt);
-    const uint32_t bw = G.bw, bh = G.bh;
-    const uint32_t bx0 = (lt % G.tiles_x) * TBW, by0 = (lt / G.tiles_x) * TBH;
-    uint32_t bx, by;
-    B::block(j, bx0, by0, bx, by);
-    if (bx >= bw || by >= bh) return;
-    uint32_t *P = G.P;
-    const uint32_t vb = P[by * bw + bx];
-    const uint32_t m = vb & 15u;
-    if (!m) return;
-    // earlier neighbours: left, up-left, up, up-right
-    const int ndx[4] = {-1, -1, 0, 1}, ndy[4] = {0, -1, -1, -1};
+    const Geom2D *G = nullptr;
+    uint32_t bx = 0, by = 0, bx0 = 0, by0 = 0, vb = 0;
+    const bool valid = border_job<TBW, TBH>(src, blockIdx.x * blockDim.x + threadIdx.x, total_tiles, G, bx, by,
+                                            bx0, by0, vb);
+    // earlier neighbours: left, up-left, up, up-right; distinct (own root, neighbour root) pairs
+    uint32_t pr[4], np = 0;
+    if (valid) {
+        const uint32_t bw = G->bw, bh = G->bh, m = vb & 15u;
+        const int ndx[4] = {-1, -1, 0, 1}, ndy[4] = {0, -1, -1, -1};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t nx = bx + ndx[k], ny = by + ndy[k];   // unsigned wrap -> out of range
-        if (nx >= bw || ny >= bh) continue;
-        if (nx - bx0 < TBW && ny - by0 < TBH) continue;    // same tile: done in the tile kernel
-        const uint32_t vn = P[ny * bw + nx];
-        const uint32_t nm = vn & 15u;
-        const bool adj = k == 0 ? adj_left(m, nm) : (k == 1 ? adj_upleft(m, nm) : (k == 2 ? adj_up(m, nm) : adj_upright(m, nm)));
-        if (adj) Forest::unite(P, vb >> 4, vn >> 4);   // tile-local roots
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t nx = bx + ndx[k], ny = by + ndy[k];   // unsigned wrap -> out of range
+            if (nx >= bw || ny >= bh) continue;
+            if (nx - bx0 < TBW && ny - by0 < TBH) continue;    // same tile: done in the tile kernel
+            const uint32_t vn = G->P[ny * bw + nx];
+            const uint32_t nm = vn & 15u;
+            const bool adj = k == 0 ? adj_left(m, nm)
+                                    : (k == 1 ? adj_upleft(m, nm) : (k == 2 ? adj_up(m, nm) : adj_upright(m, nm)));
+            if (adj && (np == 0 || pr[np - 1] != (vn >> 4))) pr[np++] = vn >> 4;
+        }
     }
+    const uint64_t key = np ? ((uint64_t)(vb >> 4) << 32) | pr[0] : 0;
+    uint32_t leader;
+    const bool lead = run_leader(key, valid ? G->tile_base : 0u, np != 0, leader);
+    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) Forest::unite(G->P, vb >> 4, pr[i]);   // tile-local roots
 }
 
+// Point each distinct tile-local root seen on the tile border (and the border
+// block itself) at its final root.
 template <uint32_t TBW, uint32_t TBH, class Src>
 __global__ __launch_bounds__(256) void bleq2d_flatten(Src src, uint32_t total_tiles)
 {
-    using B = Border<TBW, TBH>;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t gt = gid / B::JOBS, j = gid % B::JOBS;
-    if (gt >= total_tiles) return;
-    uint32_t lt;
-    const Geom2D &G = src.by_tile(gt, lt);
-    const uint32_t bx0 = (lt % G.tiles_x) * TBW, by0 = (lt / G.tiles_x) * TBH;
-    uint32_t bx, by;
-    B::block(j, bx0, by0, bx, by);
-    if (bx >= G.bw || by >= G.bh) return;
-    const uint32_t b = by * G.bw + bx;
-    if (G.P[b] & 15u) Forest::compress(G.P, b);
+    const Geom2D *G = nullptr;
+    uint32_t bx = 0, by = 0, bx0 = 0, by0 = 0, vb = 0;
+    const bool valid = border_job<TBW, TBH>(src, blockIdx.x * blockDim.x + threadIdx.x, total_tiles, G, bx, by,
+                                            bx0, by0, vb);
+    const uint32_t r = vb >> 4;
+    uint32_t leader;
+    const bool lead = run_leader(r, valid ? G->tile_base : 0u, valid, leader);
+    uint32_t root = 0;
+    if (lead) {
+        uint32_t *P = G->P;
+        const uint32_t vr = dev_load(&P[r]);
+        root = vr >> 4;
+        if (root != r) {
+            root = Forest::find(P, r);
+            dev_min(&P[r], (root << 4) | (vr & 15u));
+        }
+    }
+    root = __shfl(root, leader, 64);
+    if (valid && root != r) {
+        const uint32_t b = by * G->bw + bx;
+        if (b != r) dev_min(&G->P[b], (root << 4) | (vb & 15u));   // border block one hop from its root
+    }
 }
 
 // ---- 4. final kernel -------------------------------------------------------------------
@@ -375,7 +418,15 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
         store_row8(row0, top, x0, w);
         if (two_rows) store_row8(row1, bot, x0, w);
     }
-    if (G.count) wave_add_atomic(G.count, roots);   // one image per workgroup: uniform branch
+    if (G.count) count_add(G.slots, roots);   // one image per workgroup: uniform branch
+}
+
+template <class Src>
+__global__ __launch_bounds__(kCountSlots) void bleq2d_count(Src src)
+{
+    uint32_t li;
+    const Geom2D &G = src.by_image(blockIdx.x, li);
+    if (G.count) count_reduce(G.slots, G.count);
 }
 
 // ---- host side --------------------------------------------------------------------
@@ -399,6 +450,10 @@ struct Variant {
         }
         hipLaunchKernelGGL((bleq2d_final<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
         if (kt) kt->mark(s, "bleq2d_final");
+        if (src.images()) {
+            hipLaunchKernelGGL((bleq2d_count<Src>), dim3(src.images()), dim3(kCountSlots), 0, s, src);
+            if (kt) kt->mark(s, "bleq2d_count");
+        }
         return hipGetLastError();
     }
 };
@@ -438,7 +493,7 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
 size_t bleq2d_workspace(uint32_t h, uint32_t w)
 {
     const uint64_t bw = (w + 1ull) / 2, bh = (h + 1ull) / 2;
-    return align256(bw * bh * sizeof(uint32_t));
+    return align256(bw * bh * sizeof(uint32_t)) + kCountBytes;
 }
 
 hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
@@ -455,6 +510,7 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
     g.P = static_cast<uint32_t *>(ws);
     g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
     bleq2d_geometry(g, opt.tile2d);
+    g.slots = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256((size_t)g.bw * g.bh * 4));
     if (kt) kt->begin(s);
     return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten);
 }

@@ -110,7 +110,7 @@ template <bool VEC>
 __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict__ vox, uint32_t d, uint32_t h,
                                                         uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
                                                         uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P,
-                                                        uint8_t *__restrict__ M, uint32_t *count)
+                                                        uint8_t *__restrict__ M, uint32_t *count, uint32_t *slots)
 {
     __shared__ uint32_t par[TILE_BLOCKS];
     __shared__ uint2 mw[TZ][TY][SEGS + 2];   // guard column on each side
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict
     const uint32_t tx = blockIdx.x % tiles_x, ty = (blockIdx.x / tiles_x) % tiles_y, tz = blockIdx.x / (tiles_x * tiles_y);
     const uint32_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * TZ;
     const uint32_t bxs = bx0 + s * SEG, by = by0 + ly, bz = bz0 + lz;
-    if (blockIdx.x == 0 && t == 0 && count) *count = 0;
+    if (blockIdx.x == 0 && count) count_clear(slots);
 
     uint32_t lo = 0, hi = 0;
     if (bz < bd && by < bh && bxs < bw) {
@@ -307,7 +307,7 @@ template <bool VEC>
 __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
                                                           uint32_t *__restrict__ labels, uint32_t d, uint32_t h,
                                                           uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
-                                                          uint32_t xgroups, uint32_t *count)
+                                                          uint32_t xgroups, uint32_t *count, uint32_t *slots)
 {
     const uint32_t row = blockIdx.x / xgroups;           // block row index bz * bh + by
     const uint32_t by = row % bh, bz = row / bh;
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
             }
         }
     }
-    if (count) wave_add_atomic(count, roots);
+    if (count) count_add(slots, roots);
 }
 
 }  // namespace
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
 size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w)
 {
     const uint64_t nb = ((d + 1ull) / 2) * ((h + 1ull) / 2) * ((w + 1ull) / 2);
-    return align256(nb * sizeof(uint32_t)) + align256(nb);
+    return align256(nb * sizeof(uint32_t)) + align256(nb) + kCountBytes;
 }
 
 hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t s, KernelTimer *kt)
@@ -381,6 +381,7 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     g.ntiles = g.tiles_x * g.tiles_y * tiles_z;
     uint32_t *P = static_cast<uint32_t *>(ws);
     uint8_t *M = static_cast<uint8_t *>(ws) + align256(nb * sizeof(uint32_t));
+    uint32_t *slots = reinterpret_cast<uint32_t *>(M + align256(nb));
     const bool vec_in = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
     const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
     const uint32_t face_grid = (uint32_t)(((uint64_t)g.ntiles * JOBS + 255) / 256);
@@ -389,10 +390,10 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     if (kt) kt->begin(s);
     if (vec_in)
         hipLaunchKernelGGL(bleq3d_tile<true>, dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h, im.w, bd, bh,
-                           bw, g.tiles_x, g.tiles_y, P, M, im.count);
+                           bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
     else
         hipLaunchKernelGGL(bleq3d_tile<false>, dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h, im.w, bd, bh,
-                           bw, g.tiles_x, g.tiles_y, P, M, im.count);
+                           bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_tile");
     hipLaunchKernelGGL(bleq3d_merge, dim3(face_grid), dim3(256), 0, s, P, M, g);
     if (kt) kt->mark(s, "bleq3d_merge");
@@ -401,11 +402,15 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
     if (vec_out)
         hipLaunchKernelGGL(bleq3d_final<true>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd, bh,
-                           bw, xgroups, im.count);
+                           bw, xgroups, im.count, slots);
     else
         hipLaunchKernelGGL(bleq3d_final<false>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd,
-                           bh, bw, xgroups, im.count);
+                           bh, bw, xgroups, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_final");
+    if (im.count) {
+        hipLaunchKernelGGL(count_reduce_kernel<0>, dim3(1), dim3(kCountSlots), 0, s, slots, im.count);
+        if (kt) kt->mark(s, "count_reduce");
+    }
     return hipGetLastError();
 }
 

@@ -40,12 +40,12 @@ constexpr uint32_t JOBS = 2 * TW + 2 * (TH - 2);
 template <bool VEC, int CONN>
 __global__ __launch_bounds__(NT) void lbeq2d_tile(const uint8_t *__restrict__ px, size_t pitch, uint32_t h,
                                                   uint32_t w, uint32_t tiles_x, uint32_t *__restrict__ L,
-                                                  uint32_t *count, uint32_t cap, uint32_t *status)
+                                                  uint32_t *slots, uint32_t cap, uint32_t *status)
 {
     __shared__ uint32_t S[TH * TW];
     const uint32_t t = threadIdx.x;
     const uint32_t x0 = (blockIdx.x % tiles_x) * TW, y0 = (blockIdx.x / tiles_x) * TH;
-    if (blockIdx.x == 0 && t == 0 && count) *count = 0;
+    if (blockIdx.x == 0 && slots) count_clear(slots);
 
     // load: thread -> 16 consecutive pixels of one row; init label = local index + 1
     {
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void lbeq2d_flatten(uint32_t *L, uint32_t h, u
 // ---- 4. final analyze pass -------------------------------------------------------
 
 template <bool VEC>
-__global__ __launch_bounds__(256) void lbeq_final(uint32_t *__restrict__ L, uint64_t n, uint32_t *count)
+__global__ __launch_bounds__(256) void lbeq_final(uint32_t *__restrict__ L, uint64_t n, uint32_t *slots)
 {
     const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     uint32_t roots = 0;
@@ -235,37 +235,46 @@ __global__ __launch_bounds__(256) void lbeq_final(uint32_t *__restrict__ L, uint
                     if (i0 + k < n) L[i0 + k] = v[k];
         }
     }
-    if (count) wave_add_atomic(count, roots);
+    if (slots) count_add(slots, roots);
 }
 
 }  // namespace
 
-size_t lbeq2d_workspace(uint32_t, uint32_t) { return 256; }
+size_t lbeq2d_workspace(uint32_t, uint32_t) { return kCountBytes; }
 
-// Shared with the 3D label-equivalence path: the analyze pass over n labels.
-hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, hipStream_t s)
+// Shared with the 3D label-equivalence path: the analyze pass over n labels,
+// then the component count (slots -> *count) when count is not null.
+hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, uint32_t *slots, hipStream_t s,
+                             KernelTimer *kt)
 {
     const uint64_t threads = (n + 3) / 4;
     const dim3 grid((uint32_t)((threads + 255) / 256));
+    uint32_t *sl = count ? slots : nullptr;
     if (reinterpret_cast<uintptr_t>(labels) % 16 == 0)
-        hipLaunchKernelGGL(lbeq_final<true>, grid, dim3(256), 0, s, labels, n, count);
+        hipLaunchKernelGGL(lbeq_final<true>, grid, dim3(256), 0, s, labels, n, sl);
     else
-        hipLaunchKernelGGL(lbeq_final<false>, grid, dim3(256), 0, s, labels, n, count);
+        hipLaunchKernelGGL(lbeq_final<false>, grid, dim3(256), 0, s, labels, n, sl);
+    if (kt) kt->mark(s, "lbeq_final");
+    if (count) {
+        hipLaunchKernelGGL(count_reduce_kernel<0>, dim3(1), dim3(kCountSlots), 0, s, slots, count);
+        if (kt) kt->mark(s, "count_reduce");
+    }
     return hipGetLastError();
 }
 
-hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *, hipStream_t s, KernelTimer *kt)
+hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
 {
     const uint32_t tiles_x = (im.w + TW - 1) / TW, tiles_y = (im.h + TH - 1) / TH;
     const uint32_t ntiles = tiles_x * tiles_y;
     const bool vec = (im.pitch % 16 == 0) && (reinterpret_cast<uintptr_t>(im.px) % 16 == 0);
     const uint32_t border_grid = (uint32_t)(((uint64_t)ntiles * JOBS + 255) / 256);
+    uint32_t *slots = static_cast<uint32_t *>(ws);
 
     if (kt) kt->begin(s);
     auto tile = conn == 8 ? (vec ? lbeq2d_tile<true, 8> : lbeq2d_tile<false, 8>)
                           : (vec ? lbeq2d_tile<true, 4> : lbeq2d_tile<false, 4>);
     hipLaunchKernelGGL(tile, dim3(ntiles), dim3(NT), 0, s, im.px, im.pitch, im.h, im.w, tiles_x, im.labels,
-                       im.count, opt.iter_cap, opt.status);
+                       im.count ? slots : nullptr, opt.iter_cap, opt.status);
     if (kt) kt->mark(s, "lbeq2d_tile");
     if (conn == 8)
         hipLaunchKernelGGL(lbeq2d_merge<8>, dim3(border_grid), dim3(256), 0, s, im.labels, im.h, im.w, tiles_x, ntiles);
@@ -274,8 +283,7 @@ hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *, h
     if (kt) kt->mark(s, "lbeq2d_merge");
     hipLaunchKernelGGL(lbeq2d_flatten, dim3(border_grid), dim3(256), 0, s, im.labels, im.h, im.w, tiles_x, ntiles);
     if (kt) kt->mark(s, "lbeq2d_flatten");
-    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.h * im.w, im.count, s);
-    if (kt) kt->mark(s, "lbeq_final");
+    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.h * im.w, im.count, slots, s, kt);
     return e != hipSuccess ? e : hipGetLastError();
 }
 

@@ -19,7 +19,8 @@
 
 namespace lbl {
 
-hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, hipStream_t s);
+hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, uint32_t *slots, hipStream_t s,
+                             KernelTimer *kt);
 
 namespace {
 
@@ -33,13 +34,13 @@ constexpr uint32_t JOBS = 2 * FACE_Z + 2 * FACE_Y + 2 * FACE_X;
 template <bool VEC>
 __global__ __launch_bounds__(NT) void lbeq3d_tile(const uint8_t *__restrict__ vox, uint32_t d, uint32_t h, uint32_t w,
                                                   uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ L,
-                                                  uint32_t *count, uint32_t cap, uint32_t *status)
+                                                  uint32_t *slots, uint32_t cap, uint32_t *status)
 {
     __shared__ uint32_t S[TILE];
     const uint32_t t = threadIdx.x;
     const uint32_t x0 = (blockIdx.x % tiles_x) * TX, y0 = ((blockIdx.x / tiles_x) % tiles_y) * TY;
     const uint32_t z0 = (blockIdx.x / (tiles_x * tiles_y)) * TZ;
-    if (blockIdx.x == 0 && t == 0 && count) *count = 0;
+    if (blockIdx.x == 0 && slots) count_clear(slots);
 
     {   // load: thread -> 16 voxels of one row
         const uint32_t row = t / (TX / 16), xs = (t % (TX / 16)) * 16;
@@ -192,28 +193,28 @@ __global__ __launch_bounds__(256) void lbeq3d_flatten(uint32_t *L, Grid3 g)
 
 }  // namespace
 
-size_t lbeq3d_workspace(uint32_t, uint32_t, uint32_t) { return 256; }
+size_t lbeq3d_workspace(uint32_t, uint32_t, uint32_t) { return kCountBytes; }
 
-hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *, hipStream_t s, KernelTimer *kt)
+hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
 {
     Grid3 g{im.d, im.h, im.w, (im.w + TX - 1) / TX, (im.h + TY - 1) / TY, 0};
     g.ntiles = g.tiles_x * g.tiles_y * ((im.d + TZ - 1) / TZ);
     const bool vec = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
     const uint32_t face_grid = (uint32_t)(((uint64_t)g.ntiles * JOBS + 255) / 256);
+    uint32_t *slots = static_cast<uint32_t *>(ws);
     if (kt) kt->begin(s);
     if (vec)
         hipLaunchKernelGGL(lbeq3d_tile<true>, dim3(g.ntiles), dim3(NT), 0, s, im.vox, im.d, im.h, im.w, g.tiles_x,
-                           g.tiles_y, im.labels, im.count, opt.iter_cap, opt.status);
+                           g.tiles_y, im.labels, im.count ? slots : nullptr, opt.iter_cap, opt.status);
     else
         hipLaunchKernelGGL(lbeq3d_tile<false>, dim3(g.ntiles), dim3(NT), 0, s, im.vox, im.d, im.h, im.w, g.tiles_x,
-                           g.tiles_y, im.labels, im.count, opt.iter_cap, opt.status);
+                           g.tiles_y, im.labels, im.count ? slots : nullptr, opt.iter_cap, opt.status);
     if (kt) kt->mark(s, "lbeq3d_tile");
     hipLaunchKernelGGL(lbeq3d_merge, dim3(face_grid), dim3(256), 0, s, im.labels, g);
     if (kt) kt->mark(s, "lbeq3d_merge");
     hipLaunchKernelGGL(lbeq3d_flatten, dim3(face_grid), dim3(256), 0, s, im.labels, g);
     if (kt) kt->mark(s, "lbeq3d_flatten");
-    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.d * im.h * im.w, im.count, s);
-    if (kt) kt->mark(s, "lbeq_final");
+    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.d * im.h * im.w, im.count, slots, s, kt);
     return e != hipSuccess ? e : hipGetLastError();
 }
 

@@ -171,6 +171,7 @@ void lbl_server::launch(Group *g)
             gm.w = q.w;
             gm.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
             bleq2d_geometry(gm, 0);
+            gm.slots = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(d_P) + align256((size_t)gm.bw * gm.bh * 4));
             gm.tile_base = tb;
             gm.final_base = fb;
             tb += gm.ntiles;

@@ -49,11 +49,17 @@ __device__ __forceinline__ uint32_t lds_min(uint32_t *p, uint32_t v)
 
 // ---- LDS forest: par[x] holds the parent index ------------------------------
 
+// Root of x with path halving: x's entry is lowered to its grandparent (by
+// atomicMin, so an entry never grows) before stepping there.  Runs of a
+// dense tile link into long chains without it.
 __device__ __forceinline__ uint32_t lds_find(uint32_t *par, uint32_t x)
 {
     uint32_t p = lds_load(&par[x]);
     while (p != x) {
-        x = p;
+        const uint32_t g = lds_load(&par[p]);
+        if (g == p) return p;
+        lds_min(&par[x], g);
+        x = g;
         p = lds_load(&par[x]);
     }
     return x;
@@ -208,11 +214,65 @@ __device__ __forceinline__ void store_nt2(uint32_t *p, uint32_t a, uint32_t b)
 
 // ---- wave helpers ----------------------------------------------------------
 
-// Add `v` over the workgroup's waves into *dst with one atomic per wave.
-__device__ __forceinline__ void wave_add_atomic(uint32_t *dst, uint32_t v)
+// Component counting.  One counter word hit by every wave serialises tens of
+// thousands of atomics on one L2 line, so counts go to kCountSlots words 64 B
+// apart (a workgroup adds its total to slot blockIdx % kCountSlots) and
+// count_reduce sums the slots afterwards.
+constexpr uint32_t kCountSlots = 64;
+constexpr uint32_t kCountStride = 16;   // words between slots
+constexpr size_t kCountBytes = kCountSlots * kCountStride * sizeof(uint32_t);
+
+// Workgroup-wide sum of v, added to slots[blockIdx % kCountSlots] by thread 0.
+// Every thread of the workgroup must call it.
+__device__ __forceinline__ void count_add(uint32_t *slots, uint32_t v)
 {
+    __shared__ uint32_t part[16];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63u) == 0 && v) atomicAdd(dst, v);
+    const uint32_t wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63u) == 0) part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t i = 0; i < nwaves; ++i) s += part[i];
+        if (s) atomicAdd(&slots[(blockIdx.x % kCountSlots) * kCountStride], s);
+    }
+}
+
+// Sum the slots into *count (called by one wave of kCountSlots threads).
+__device__ __forceinline__ void count_reduce(const uint32_t *slots, uint32_t *count)
+{
+    uint32_t v = threadIdx.x < kCountSlots ? slots[threadIdx.x * kCountStride] : 0u;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (threadIdx.x == 0) *count = v;
+}
+
+template <int Dummy = 0>
+__global__ __launch_bounds__(kCountSlots) void count_reduce_kernel(const uint32_t *slots, uint32_t *count)
+{
+    count_reduce(slots, count);
+}
+
+// First lane of a run of consecutive lanes holding the same (domain, key), and
+// that lane's index (lanes whose `active` is false never lead and never match;
+// domain separates images of a batch, whose indices overlap).  All lanes of
+// the wave must call it.
+__device__ __forceinline__ bool run_leader(uint64_t key, uint32_t domain, bool active, uint32_t &leader_lane)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t prev = __shfl_up(key, 1, 64);
+    const uint32_t prev_dom = __shfl_up(domain, 1, 64);
+    const bool prev_active = __shfl_up((int)active, 1, 64) != 0;
+    const bool lead = active && (lane == 0 || !prev_active || prev != key || prev_dom != domain);
+    const uint64_t leaders = __ballot(lead);
+    const uint64_t below = leaders & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    leader_lane = below ? 63u - (uint32_t)__clzll(below) : lane;
+    return lead;
+}
+
+// Zero the slots (called by one workgroup before any count_add).
+__device__ __forceinline__ void count_clear(uint32_t *slots)
+{
+    if (threadIdx.x < kCountSlots) slots[threadIdx.x * kCountStride] = 0;
 }
 
 // Nonzero-byte test: bit 8k of the result is set iff byte k of v is nonzero.
