@@ -98,6 +98,13 @@ def bench_2d(L, torch, lab, a):
         x = d_in[i % len(d_in)]
         lab.label2d_dev(x.data_ptr(), side, side, side, d_lab.data_ptr(), alg, conn, d_n.data_ptr(), sptr)
 
+    tuning = None
+    if not a.no_autotune:
+        # the library's autotuner: tile shape and merge schedule for this size class,
+        # timed on the densest-component input (p=0.5)
+        log("[bench] autotune")
+        lab.autotune2d(d_in[2].data_ptr(), side, side, side, d_lab.data_ptr(), sptr)
+        tuning = lab.tuning()
     log(f"[bench] 2D timed {a.steps} steps")
     ms, wall = timed_steps(torch, stream, step, len(inputs), a.warmup, a.steps)
     names = [n for n, _ in inputs]
@@ -109,6 +116,7 @@ def bench_2d(L, torch, lab, a):
     rl["path_us"] = round(path_us, 2)
     rl["kernels_us"] = {k: round(v, 2) for k, v in kavg.items()}
     rl["per_input_us"] = {k: round(v, 2) for k, v in per_input.items()}
+    rl["autotuned"] = tuning
     out = {"value": px * a.steps / (ms / 1e3) / 1e9, "ms_per_step": ms / a.steps, "wall_s": wall,
            "roofline": rl, "inputs": inputs}
 
@@ -333,6 +341,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline 2D line only")
     ap.add_argument("--skip", default="", help="comma list of extras to skip: 3d,batch,serve")
     ap.add_argument("--serve-qps", type=float, default=200.0)

@@ -236,6 +236,19 @@ class Labeler:
         n = self.lib.lbl_last_kernel_times(self._ctx, us, names, 16)
         return [(names[i].decode(), float(us[i])) for i in range(max(n, 0))]
 
+    def autotune2d(self, d_px: int, h: int, w: int, pitch: int, d_labels: int, stream: Optional[int] = None,
+                   alg="bleq"):
+        self._check(self.lib.lbl_autotune2d(self._ctx, _alg_id(alg), d_px, h, w, pitch, d_labels, stream or None))
+
+    def tuning(self) -> dict:
+        """The context's tuning table ({"alg/kind/size class": [tile, flatten, us]})."""
+        import json
+        import tempfile
+
+        with tempfile.NamedTemporaryFile("r", suffix=".json") as f:
+            self._check(self.lib.lbl_tuning_save(self._ctx, f.name.encode()))
+            return json.load(open(f.name))
+
     def label2d(self, img: np.ndarray, alg="bleq", conn: int = 0) -> Tuple[np.ndarray, int, float]:
         img = np.ascontiguousarray(img, dtype=np.uint8)
         if img.ndim != 2:

@@ -116,6 +116,7 @@ __global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
 {
     constexpr uint32_t SEGS = TBW / SEG;
     __shared__ uint32_t par[TBH * TBW];
+    const SwizzleMap<SEG, (TBW / SEG) * TBH> at;   // conflict-free placement of the forest
     __shared__ uint32_t mw[TBH][SEGS + 2];   // 8 nibbles per thread, guard column on each side
 
     uint32_t lt;
@@ -167,9 +168,9 @@ __global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
         for (uint32_t k = 0; k < SEG; ++k) {
             const uint32_t m = (m8 >> (4 * k)) & 15u, li = li0 + k;
             if (m && adj_left(m, prev)) {
-                par[li] = head;
+                par[at(li)] = head;
             } else {
-                par[li] = li;
+                par[at(li)] = li;
                 head = li;
             }
             prev = m;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
     // unions with blocks of other threads inside the tile
     if (s > 0) {
         const uint32_t m0 = m8 & 15u, left = mw[r][s] >> 28;
-        if (adj_left(m0, left)) lds_union(par, li0, li0 - 1);
+        if (adj_left(m0, left)) lds_union(par, li0, li0 - 1, at);
     }
     if (r > 0) {
         const uint32_t up = mw[r - 1][s + 1];
@@ -210,7 +211,7 @@ __global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
                         continue;
                     }
                 }
-                lds_union(par, li0 + k, (uint32_t)((int)ubase + c));
+                lds_union(par, li0 + k, (uint32_t)((int)ubase + c), at);
                 last = c;
             }
         }
@@ -223,7 +224,7 @@ __global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
 #pragma unroll
         for (uint32_t k = 0; k < SEG; ++k) {
             const uint32_t m = (m8 >> (4 * k)) & 15u, li = li0 + k;
-            const uint32_t root = m ? lds_find(par, li) : li;
+            const uint32_t root = m ? lds_find(par, li, at) : li;
             const uint32_t g = (by0 + root / TBW) * bw + bx0 + root % TBW;
             out[k] = (g << 4) | m;
         }
@@ -467,13 +468,16 @@ hipError_t dispatch(uint32_t tile_id, const Src &src, uint32_t total_tiles, uint
     case 2: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
     case 3: return Variant<256, 8>::launch(src, total_tiles, total_final, s, kt, flatten);
     case 4: return Variant<64, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 5: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 6: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
     default: return Variant<128, 16>::launch(src, total_tiles, total_final, s, kt, flatten);
     }
 }
 
 }  // namespace
 
-const TileShape2D kBleq2dTiles[kBleq2dTileCount] = {{128, 16}, {64, 16}, {128, 32}, {256, 8}, {64, 32}};
+const TileShape2D kBleq2dTiles[kBleq2dTileCount] = {{128, 16}, {64, 16}, {128, 32}, {256, 8},
+                                                    {64, 32},  {128, 64}, {256, 32}};
 
 void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
 {

@@ -33,7 +33,7 @@ struct TileShape2D {
     uint32_t bw, bh;   // tile size in blocks
 };
 
-constexpr uint32_t kBleq2dTileCount = 5;
+constexpr uint32_t kBleq2dTileCount = 7;
 extern const TileShape2D kBleq2dTiles[kBleq2dTileCount];
 
 // Fill bw..nfinal and the alignment flags for a tile shape (px, labels, pitch,

@@ -113,6 +113,7 @@ __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict
                                                         uint8_t *__restrict__ M, uint32_t *count, uint32_t *slots)
 {
     __shared__ uint32_t par[TILE_BLOCKS];
+    const SwizzleMap<SEG, NT_TILE> at;   // conflict-free placement of the forest
     __shared__ uint2 mw[TZ][TY][SEGS + 2];   // guard column on each side
 
     const uint32_t t = threadIdx.x, s = t % SEGS, ly = (t / SEGS) % TY, lz = t / (SEGS * TY);
@@ -146,9 +147,9 @@ __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict
         for (int k = 0; k < (int)SEG; ++k) {
             const uint32_t m = mask_at(lo, hi, k), li = li0 + k;
             if (m && adjacent(m, prev, -1, 0, 0)) {
-                par[li] = head;
+                par[at(li)] = head;
             } else {
-                par[li] = li;
+                par[at(li)] = li;
                 head = li;
             }
             prev = m;
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict
     if (s > 0) {
         const uint32_t m0 = mask_at(lo, hi, 0);
         const uint32_t left = mw[lz][ly][s].y >> 24;
-        if (adjacent(m0, left, -1, 0, 0)) lds_union(par, li0, li0 - 1);
+        if (adjacent(m0, left, -1, 0, 0)) lds_union(par, li0, li0 - 1, at);
     }
     // earlier neighbour rows inside the tile: (dy, dz) = (-1,0), (-1,-1), (0,-1), (+1,-1)
     const int rdy[4] = {-1, -1, 0, 1}, rdz[4] = {0, -1, -1, -1};
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict
                         continue;
                     }
                 }
-                lds_union(par, li0 + k, (uint32_t)(ubase + c));
+                lds_union(par, li0 + k, (uint32_t)(ubase + c), at);
                 last = c;
             }
         }
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict
 #pragma unroll
         for (int k = 0; k < (int)SEG; ++k) {
             const uint32_t m = mask_at(lo, hi, k), li = li0 + k;
-            const uint32_t root = m ? lds_find(par, li) : li;
+            const uint32_t root = m ? lds_find(par, li, at) : li;
             const uint32_t rx = root % TX, ry = (root / TX) % TY, rz = root / (TX * TY);
             out[k] = ((bz0 + rz) * bh + by0 + ry) * bw + bx0 + rx;
         }

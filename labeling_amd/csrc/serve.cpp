@@ -83,8 +83,9 @@ struct Req {
     uint64_t lab_bytes() const { return align256(elems() * 4); }
     uint64_t ws_bytes() const
     {
-        if (is3d()) return r.alg == LBL_BLEQ ? align256(bleq3d_workspace(r.d, r.h, r.w)) : 0;
-        return r.alg == LBL_BLEQ ? align256(bleq2d_workspace(r.h, r.w)) : 0;
+        if (is3d())
+            return align256(r.alg == LBL_BLEQ ? bleq3d_workspace(r.d, r.h, r.w) : lbeq3d_workspace(r.d, r.h, r.w));
+        return align256(r.alg == LBL_BLEQ ? bleq2d_workspace(r.h, r.w) : lbeq2d_workspace(r.h, r.w));
     }
     uint64_t bytes() const { return in_bytes() + lab_bytes() + ws_bytes(); }
 };

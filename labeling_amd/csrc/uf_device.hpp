@@ -47,32 +47,48 @@ __device__ __forceinline__ uint32_t lds_min(uint32_t *p, uint32_t v)
     return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// ---- LDS forest: par[x] holds the parent index ------------------------------
+// ---- LDS forest: parent of element x stored at par[Map(x)] ------------------
+
+// Identity placement.
+struct LinearMap {
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return x; }
+};
+
+// Element x = SEG * t + k (block k of thread t) stored at k * NT + t: when a
+// wave touches block k of each of its threads the 64 words are consecutive,
+// i.e. in distinct LDS banks.  With x in place, the 64 lanes would hit
+// 32 / SEG banks only.
+template <uint32_t SEG, uint32_t NT>
+struct SwizzleMap {
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return (x % SEG) * NT + x / SEG; }
+};
 
 // Root of x with path halving: x's entry is lowered to its grandparent (by
 // atomicMin, so an entry never grows) before stepping there.  Runs of a
 // dense tile link into long chains without it.
-__device__ __forceinline__ uint32_t lds_find(uint32_t *par, uint32_t x)
+template <class Map = LinearMap>
+__device__ __forceinline__ uint32_t lds_find(uint32_t *par, uint32_t x, Map at = Map())
 {
-    uint32_t p = lds_load(&par[x]);
+    uint32_t p = lds_load(&par[at(x)]);
     while (p != x) {
-        const uint32_t g = lds_load(&par[p]);
+        const uint32_t g = lds_load(&par[at(p)]);
         if (g == p) return p;
-        lds_min(&par[x], g);
+        lds_min(&par[at(x)], g);
         x = g;
-        p = lds_load(&par[x]);
+        p = lds_load(&par[at(x)]);
     }
     return x;
 }
 
-__device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b)
+template <class Map = LinearMap>
+__device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b, Map at = Map())
 {
     for (;;) {
-        a = lds_find(par, a);
-        b = lds_find(par, b);
+        a = lds_find(par, a, at);
+        b = lds_find(par, b, at);
         if (a == b) return;
         if (a < b) { uint32_t t = a; a = b; b = t; }
-        uint32_t old = lds_min(&par[a], b);
+        uint32_t old = lds_min(&par[at(a)], b);
         if (old == a) return;
         a = old;
     }
@@ -91,20 +107,26 @@ struct PackedForest {
     static constexpr uint32_t kLowMask = (1u << SHIFT) - 1u;
 
     // Root of x with path halving (x -> grandparent, by atomicMin so a link
-    // never grows).
+    // never grows); the starting element is then pointed at the root, so the
+    // next find from it (typically another border union of the same tile-local
+    // root) takes one hop.
     __device__ static uint32_t find(uint32_t *w, uint32_t x)
     {
-        uint32_t v = dev_load(&w[x]);
+        const uint32_t x0 = x;
+        const uint32_t v0 = dev_load(&w[x]);
+        uint32_t v = v0, root;
         for (;;) {
-            uint32_t p = v >> SHIFT;
-            if (p == x) return x;
-            uint32_t vp = dev_load(&w[p]);
-            uint32_t gp = vp >> SHIFT;
-            if (gp == p) return p;
+            const uint32_t p = v >> SHIFT;
+            if (p == x) { root = x; break; }
+            const uint32_t vp = dev_load(&w[p]);
+            const uint32_t gp = vp >> SHIFT;
+            if (gp == p) { root = p; break; }
             dev_min(&w[x], (gp << SHIFT) | (v & kLowMask));
             x = gp;
             v = dev_load(&w[x]);
         }
+        if (root != x0 && root != (v0 >> SHIFT)) dev_min(&w[x0], (root << SHIFT) | (v0 & kLowMask));
+        return root;
     }
 
     // Root of x when nothing writes the forest any more (plain cached loads).

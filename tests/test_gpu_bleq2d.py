@@ -167,3 +167,48 @@ def test_full_size_vs_scipy_golden(labeler, name):
     c, m = O.canonical(lab)
     assert hashlib.sha256(c.tobytes()).hexdigest() == entry["conn8"]["labels_sha256"]
     assert n == m == entry["conn8"]["components"]
+
+
+@pytest.mark.parametrize("flatten", [1, 0])
+@pytest.mark.parametrize("tile", range(7))
+def test_every_tile_shape_and_schedule(tmp_path, tile, flatten):
+    """Each launch configuration the autotuner can pick gives the same labels."""
+    import math
+
+    lab = L.Labeler(0)   # own context: its tuning table forces the configuration
+    try:
+        for k, (h, w) in enumerate([(517, 1029), (1024, 1024), (33, 255), (300, 2047)]):
+            cls = int(math.floor(math.log2(h * w)))
+            path = tmp_path / f"t{k}.json"
+            path.write_text(json.dumps({f"bleq/single2d/{cls}": [tile, flatten, 0.0]}))
+            assert lab.lib.lbl_tuning_load(lab.handle, str(path).encode()) == L.OK
+            img = L.bernoulli((h, w), 0.3 + 0.1 * k, seed=50 + k)
+            got, n, _ = lab.label2d(img, "bleq")
+            canon, m = O.bfs2d(img, 8)
+            np.testing.assert_array_equal(O.canonical(got)[0], canon, err_msg=f"tile {tile} {h}x{w}")
+            assert n == m
+            assert lab.tuning()[f"bleq/single2d/{cls}"][:2] == [tile, flatten]
+    finally:
+        lab.close()
+
+
+def test_autotune_then_label(tmp_path):
+    from tests import hiprt
+
+    lab = L.Labeler(0)
+    try:
+        h, w = 1500, 2300
+        img = L.bernoulli((h, w), 0.5, seed=77)
+        d_px = hiprt.to_device(img)
+        d_lab = hiprt.DeviceBuffer(h * w * 4)
+        lab.autotune2d(d_px.addr, h, w, w, d_lab.addr)
+        table = lab.tuning()
+        assert any(k.startswith("bleq/single2d/") for k in table), table
+        got, n, _ = lab.label2d(img, "bleq")
+        canon, m = O.bfs2d(img, 8)
+        np.testing.assert_array_equal(O.canonical(got)[0], canon)
+        p = tmp_path / "tuning.json"
+        assert lab.lib.lbl_tuning_save(lab.handle, str(p).encode()) == L.OK
+        assert json.loads(p.read_text()) == table
+    finally:
+        lab.close()
