@@ -99,7 +99,16 @@ def bench_2d(L, torch, lab, a):
         lab.label2d_dev(x.data_ptr(), side, side, side, d_lab.data_ptr(), alg, conn, d_n.data_ptr(), sptr)
 
     tuning = None
-    if not a.no_autotune:
+    if a.variant:
+        # a fixed kernel variant (tile shape id, flatten 0/1) for this size class
+        import tempfile
+        tile, flat = (int(v) for v in a.variant.split(","))
+        with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as f:
+            json.dump({f"bleq/single2d/{int(math.floor(math.log2(side * side)))}": [tile, flat, 0.0]}, f)
+        lab._check(lab.lib.lbl_tuning_load(lab.handle, f.name.encode()))
+        os.unlink(f.name)
+        tuning = lab.tuning()
+    elif not a.no_autotune:
         # the library's autotuner: tile shape and merge schedule for this size class,
         # timed on the densest-component input (p=0.5)
         log("[bench] autotune")
@@ -342,6 +351,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--variant", default="", help="TILE,FLATTEN: run this bleq 2D kernel variant, no autotune")
     ap.add_argument("--quick", action="store_true", help="headline 2D line only")
     ap.add_argument("--skip", default="", help="comma list of extras to skip: 3d,batch,serve")
     ap.add_argument("--serve-qps", type=float, default=200.0)

@@ -36,8 +36,8 @@
 namespace lbl {
 namespace {
 
-constexpr uint32_t SEG = 8;              // blocks per thread in the tile kernel
-constexpr uint32_t FINAL_BLOCKS = 4;     // blocks per thread in the final kernel
+constexpr uint32_t FINAL_BLOCKS = 2;     // blocks per thread in the final kernel: one 16-byte store per
+                                         // pixel row, so each wave store covers 1 KiB contiguously
 constexpr uint32_t NT_FINAL = 256;
 
 using Forest = PackedForest<4>;
@@ -110,131 +110,254 @@ struct Many {
 };
 
 // ---- 1. tile kernel -------------------------------------------------------------
+//
+// A tile is TBW x TBH blocks; one thread ("lane") per 32-pixel-wide segment of
+// one band (a block row = two pixel rows), TBW / 16 lanes per band, all lanes
+// of a band in one wave.  A lane holds its segment as two 32-bit masks, `a`
+// (top row) and `b` (bottom row).  Inside a band, 8-connectivity is exactly
+// "same run of set bits of o = a | b" (two pixels in adjacent columns of a
+// 2-row band always touch), so the union-find nodes are the segment-local runs,
+// each named by the block of its first column:
+//   * a run continuing the left segment's run is linked straight to the head
+//     of the whole band run, found with a ballot over the band's lanes (the
+//     nearest segment to the left that is not all foreground), and stands for
+//     that head in every union;
+//   * vertical contacts (a top-row pixel of this band touching a bottom-row
+//     pixel of the band above, straight or diagonally) are found with bit
+//     operations; of the straight ones only the first after each lower or
+//     upper run boundary is kept (the others repeat its pair of runs).
+// Each lane feeds its contacts through one union loop (a single loop body for
+// all of a lane's contacts keeps divergent lanes in step; a finished union
+// points both of its nodes at the root, which keeps later finds short), then
+// resolves its nodes' roots and writes the 16 block words of its segment.
+
+constexpr uint32_t LANE_BLOCKS = 16;          // blocks per lane (32 pixel columns)
+constexpr uint32_t kResolved = 0x80000000u;   // LDS tag: entry holds the global root
+
+// bit i set <=> byte i of w is non-zero (i = 0..3)
+__device__ __forceinline__ uint32_t nz_bits4(uint32_t w)
+{
+    const uint32_t nz = (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+    return (nz * 0x00204081u) >> 28;
+}
+
+__device__ __forceinline__ uint32_t nz_bits16(uint4 q)
+{
+    return nz_bits4(q.x) | (nz_bits4(q.y) << 4) | (nz_bits4(q.z) << 8) | (nz_bits4(q.w) << 12);
+}
+
+__device__ __forceinline__ uint32_t hibit(uint32_t v) { return 31u - __builtin_clz(v); }   // v != 0
+__device__ __forceinline__ uint32_t upto(uint32_t x) { return (2u << x) - 1u; }           // bits 0..x
+
+// 32 pixels of one row as a bit mask (bit k = column x0 + k), zero past the width.
+__device__ __forceinline__ uint32_t row_bits32(const uint8_t *row, uint32_t x0, uint32_t w, bool vec)
+{
+    if (vec && x0 + 32 <= w) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(row + x0);
+        return nz_bits16(q[0]) | (nz_bits16(q[1]) << 16);
+    }
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < 32 && x0 + k < w; ++k) m |= (uint32_t)(row[x0 + k] != 0) << k;
+    return m;
+}
 
 template <uint32_t TBW, uint32_t TBH, class Src>
-__global__ __launch_bounds__((TBW / SEG) * TBH) void bleq2d_tile(Src src)
+__global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src)
 {
-    constexpr uint32_t SEGS = TBW / SEG;
-    __shared__ uint32_t par[TBH * TBW];
-    const SwizzleMap<SEG, (TBW / SEG) * TBH> at;   // conflict-free placement of the forest
-    __shared__ uint32_t mw[TBH][SEGS + 2];   // 8 nibbles per thread, guard column on each side
+    constexpr uint32_t LPB = TBW / LANE_BLOCKS;   // lanes per band
+    static_assert((TBW & (TBW - 1)) == 0 && LPB >= 1 && LPB <= 64, "TBW: power of two, 16..1024");
+    constexpr uint32_t LOG_TBW = __builtin_ctz(TBW);
+    __shared__ uint32_t par[TBH * TBW];            // node (run start block) -> parent, local indices
+    __shared__ uint32_t so[TBH + 1][LPB + 2];      // o = a | b per band/segment; guard row 0, guard columns
+    __shared__ uint32_t sb[TBH + 1][LPB + 2];      // b (bottom row)
+    __shared__ uint32_t sh[TBH + 1][LPB + 2];      // node standing for the segment's first run
 
     uint32_t lt;
     const Geom2D &G = src.by_tile(blockIdx.x, lt);
     const uint32_t h = G.h, w = G.w, bw = G.bw, bh = G.bh;
-    const uint32_t t = threadIdx.x, r = t / SEGS, s = t % SEGS;
-    const uint32_t tx = lt % G.tiles_x, ty = lt / G.tiles_x;
-    const uint32_t bx0 = tx * TBW, by0 = ty * TBH;
-    const uint32_t by = by0 + r, bxs = bx0 + s * SEG;
+    const uint32_t t = threadIdx.x, r = t / LPB, j = t % LPB;
+    const uint32_t bx0 = (lt % G.tiles_x) * TBW, by0 = (lt / G.tiles_x) * TBH;
+    const uint32_t by = by0 + r, bxs = bx0 + j * LANE_BLOCKS;
     if (lt == 0 && G.count) count_clear(G.slots);
 
-    // masks of this thread's 8 blocks
-    uint32_t m8 = 0;
+    uint32_t a = 0, b = 0;
     if (by < bh && bxs < bw) {
         const uint32_t y0 = 2 * by, x0 = 2 * bxs;
         const uint8_t *row0 = G.px + (size_t)y0 * G.pitch;
-        const uint8_t *row1 = row0 + G.pitch;
-        const bool two_rows = y0 + 1 < h;
-        if ((G.flags & kVecIn) && two_rows && x0 + 16 <= w) {
-            const uint4 a = *reinterpret_cast<const uint4 *>(row0 + x0);
-            const uint4 b = *reinterpret_cast<const uint4 *>(row1 + x0);
-            m8 = quad_masks(a.x, b.x) | (quad_masks(a.y, b.y) << 8) | (quad_masks(a.z, b.z) << 16) |
-                 (quad_masks(a.w, b.w) << 24);
-        } else {
-            for (uint32_t q = 0; q < 4; ++q) {
-                uint32_t tw = 0, bwd = 0;
-                for (uint32_t j = 0; j < 4; ++j) {
-                    const uint32_t x = x0 + 4 * q + j;
-                    if (x < w) {
-                        tw |= (uint32_t)(row0[x] != 0) << (8 * j);
-                        if (two_rows) bwd |= (uint32_t)(row1[x] != 0) << (8 * j);
-                    }
-                }
-                m8 |= quad_masks(tw, bwd) << (8 * q);
-            }
-        }
+        const bool vec = (G.flags & kVecIn) != 0;
+        a = row_bits32(row0, x0, w, vec);
+        if (y0 + 1 < h) b = row_bits32(row0 + G.pitch, x0, w, vec);
     }
-    mw[r][s + 1] = m8;
-    if (s == 0) {
-        mw[r][0] = 0;
-        mw[r][SEGS + 1] = 0;
+    const uint32_t o = a | b;
+    const uint32_t st = o & ~(o << 1);               // first column of each segment-local run
+    const uint32_t nb = r * TBW + j * LANE_BLOCKS;   // local index of the segment's first block
+
+    // head of the band run this segment's first run continues (if it does)
+    uint32_t first = nb;
+    {
+        const uint32_t lw = t & 63u, base = lw - j;   // lane of segment 0 of this band
+        const uint64_t nonfull = __ballot(o != 0xFFFFFFFFu);
+        const uint64_t below = nonfull & ((1ull << lw) - 1ull) & ~((1ull << base) - 1ull);
+        const uint32_t k = below ? 63u - (uint32_t)__builtin_clzll(below) : lw;
+        const uint32_t o_left = __shfl(o, (int)((lw + 63u) & 63u), 64);
+        const uint32_t o_k = __shfl(o, (int)k, 64);
+        if (j > 0 && (o & 1u) && (o_left >> 31)) {
+            const uint32_t sk = r * TBW + (k - base) * LANE_BLOCKS;   // segment k's first block
+            first = !below ? r * TBW : ((o_k >> 31) ? sk + (hibit(o_k & ~(o_k << 1)) >> 1) : sk + LANE_BLOCKS);
+        }
     }
 
-    // horizontal runs inside the thread's segment
-    const uint32_t li0 = r * TBW + s * SEG;
+    so[r + 1][j + 1] = o;
+    sb[r + 1][j + 1] = b;
+    sh[r + 1][j + 1] = first;
+    if (j == 0) so[r + 1][0] = sb[r + 1][0] = 0;
+    if (j == LPB - 1) so[r + 1][LPB + 1] = sb[r + 1][LPB + 1] = 0;
+    if (r == 0) {
+        so[0][j + 1] = sb[0][j + 1] = 0;
+        if (j == 0) so[0][0] = sb[0][0] = 0;
+        if (j == LPB - 1) so[0][LPB + 1] = sb[0][LPB + 1] = 0;
+    }
     {
-        uint32_t head = li0, prev = 0;
+        uint4 *p4 = reinterpret_cast<uint4 *>(&par[nb]);
+        p4[0] = make_uint4(first, nb + 1, nb + 2, nb + 3);
 #pragma unroll
-        for (uint32_t k = 0; k < SEG; ++k) {
-            const uint32_t m = (m8 >> (4 * k)) & 15u, li = li0 + k;
-            if (m && adj_left(m, prev)) {
-                par[at(li)] = head;
-            } else {
-                par[at(li)] = li;
-                head = li;
-            }
-            prev = m;
-        }
+        for (uint32_t q = 1; q < LANE_BLOCKS / 4; ++q) p4[q] = make_uint4(nb + 4 * q, nb + 4 * q + 1, nb + 4 * q + 2, nb + 4 * q + 3);
     }
     __syncthreads();
 
-    // unions with blocks of other threads inside the tile
-    if (s > 0) {
-        const uint32_t m0 = m8 & 15u, left = mw[r][s] >> 28;
-        if (adj_left(m0, left)) lds_union(par, li0, li0 - 1, at);
-    }
-    if (r > 0) {
-        const uint32_t up = mw[r - 1][s + 1];
-        const uint32_t up_l = mw[r - 1][s] >> 28;         // upper column -1
-        const uint32_t up_r = mw[r - 1][s + 2] & 15u;     // upper column 8
-        const uint32_t ubase = (r - 1) * TBW + s * SEG;   // local index of upper column 0
-        int last = -3;                                    // last upper column joined by the current run
-        uint32_t prev = 0;
-        for (int k = 0; k < (int)SEG; ++k) {
-            const uint32_t m = (m8 >> (4 * k)) & 15u;
-            if (!m) {
-                prev = 0;
-                last = -3;
+    // unions along the vertical contacts with the band above
+    {
+        const uint32_t ou = so[r][j + 1], bu = sb[r][j + 1], bul = sb[r][j], bur = sb[r][j + 2], oul = so[r][j];
+        const uint32_t stu = ou & ~(ou << 1);
+        const uint32_t nbu = nb - TBW;   // only used when a contact exists (r > 0)
+        const uint32_t su_l = oul ? hibit(oul & ~(oul << 1)) : 0u;
+        const uint32_t ul_node = su_l ? nbu - LANE_BLOCKS + (su_l >> 1) : sh[r][j];   // upper-left segment's last run
+        const uint32_t ur_node = sh[r][j + 2];                                         // upper-right segment's first run
+        const uint32_t up_first = sh[r][j + 1];
+        // straight contacts: the first column of each run of a & b_up, and of
+        // those only the first after a lower or upper run boundary (carry trick:
+        // adding the boundary bits to the complement runs each carry up to the
+        // next contact of its segment); diagonal contacts only where neither
+        // the straight contact nor the neighbouring top-row pixel exists
+        uint32_t e0, el, er;
+        {
+            const uint32_t v0 = a & bu, f0 = v0 & ~(v0 << 1), bnd = st | stu | 1u;
+            const uint32_t y = ~f0 & ~(bnd >> 1);
+            e0 = ((y + bnd) & ~y) & f0;
+            el = a & ((bu << 1) | (bul >> 31)) & ~bu & ~(a << 1);
+            er = a & ((bu >> 1) | (bur << 31)) & ~bu & ~(a >> 1);
+        }
+        uint32_t ua = 0, ub = 0, oa = 0, ob = 0, rest = e0 | el | er;
+        bool busy = false;
+        for (;;) {
+            if (!busy) {
+                if (!rest) break;
+                // next contact: upper column x-1 before x before x+1
+                const uint32_t x = __builtin_ctz(rest), bit = 1u << x;
+                const bool tl = (el & bit) != 0, t0 = (e0 & bit) != 0;
+                el ^= tl ? bit : 0u;
+                e0 ^= (!tl && t0) ? bit : 0u;
+                er ^= (!tl && !t0) ? bit : 0u;
+                rest = e0 | el | er;
+                const int p = (int)x + (tl ? -1 : (t0 ? 0 : 1));
+                const uint32_t s = hibit(st & upto(x));
+                ua = s ? nb + (s >> 1) : first;
+                if (p < 0) {
+                    ub = ul_node;
+                } else if (p > 31) {
+                    ub = ur_node;
+                } else {
+                    const uint32_t su = hibit(stu & upto((uint32_t)p));
+                    ub = su ? nbu + (su >> 1) : up_first;
+                }
+                oa = ua;
+                ob = ub;
+                busy = true;
+            }
+            const uint32_t pa = lds_load(&par[ua]), pb = lds_load(&par[ub]);
+            if (pa != ua || pb != ub) {   // not both roots yet: one hop each
+                ua = pa;
+                ub = pb;
                 continue;
             }
-            if (!adj_left(m, prev)) last = -3;            // a new run starts
-            prev = m;
-            for (int c = k - 1; c <= k + 1; ++c) {
-                const uint32_t um = c < 0 ? up_l : (c >= (int)SEG ? up_r : ((up >> (4 * c)) & 15u));
-                const bool adj = c < k ? adj_upleft(m, um) : (c == k ? adj_up(m, um) : adj_upright(m, um));
-                if (!adj || c == last) continue;
-                if (c == last + 1) {
-                    // upper blocks last and c joined horizontally: same set already
-                    const uint32_t lm = last < 0 ? up_l : ((up >> (4 * last)) & 15u);
-                    if (adj_left(um, lm)) {
-                        last = c;
-                        continue;
-                    }
+            if (ua != ub) {   // link the larger root under the smaller
+                const uint32_t hi = ua > ub ? ua : ub, lo = ua ^ ub ^ hi;
+                const uint32_t old = lds_min(&par[hi], lo);
+                if (old != hi) {   // hi stopped being a root meanwhile
+                    ua = old;
+                    ub = lo;
+                    continue;
                 }
-                lds_union(par, li0 + k, (uint32_t)((int)ubase + c), at);
-                last = c;
+                ua = lo;
             }
+            // both original nodes now point straight at the set's root (or lower)
+            lds_min(&par[oa], ua);
+            lds_min(&par[ob], ua);
+            busy = false;
         }
     }
     __syncthreads();
 
-    // resolve tile-local roots and write the block words
-    if (by < bh && bxs < bw) {
-        uint32_t out[SEG];
+    // Roots of this lane's nodes.  A node still pointing at itself is a root
+    // (its global index is g0 + block); the others are chased and their entry
+    // replaced by the tagged global root.
+    const uint32_t g0 = by * bw + bxs;   // global index of local block nb
+    uint32_t v[LANE_BLOCKS];
+    const auto load_nodes = [&]() {
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(&par[nb]);
 #pragma unroll
-        for (uint32_t k = 0; k < SEG; ++k) {
-            const uint32_t m = (m8 >> (4 * k)) & 15u, li = li0 + k;
-            const uint32_t root = m ? lds_find(par, li, at) : li;
-            const uint32_t g = (by0 + root / TBW) * bw + bx0 + root % TBW;
-            out[k] = (g << 4) | m;
+        for (uint32_t q = 0; q < LANE_BLOCKS / 4; ++q) {
+            const uint4 x = p4[q];
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
+    };
+    load_nodes();
+    uint32_t pend = 0;
+#pragma unroll
+    for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk)
+        if ((st & (3u << (2 * kk))) && v[kk] != nb + kk) pend |= 1u << kk;
+    if (pend) {
+        uint32_t cur = 0, orig = 0;
+        bool busy = false;
+        for (;;) {
+            if (!busy) {
+                if (!pend) break;
+                orig = cur = nb + __builtin_ctz(pend);
+                pend &= pend - 1;
+                busy = true;
+            }
+            const uint32_t p = lds_load(&par[cur]);
+            uint32_t g;
+            if (p & kResolved) {
+                g = p;
+            } else if (p == cur) {
+                g = kResolved | ((by0 + (cur >> LOG_TBW)) * bw + bx0 + (cur & (TBW - 1)));
+            } else {
+                cur = p;
+                continue;
+            }
+            lds_store(&par[orig], g);
+            busy = false;
+        }
+        load_nodes();
+    }
+
+    // block words: root << 4 | mask (background: own index << 4)
+    if (by < bh && bxs < bw) {
+        uint32_t out[LANE_BLOCKS], cur = 0;
+#pragma unroll
+        for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {
+            if (st & (3u << (2 * kk))) cur = (v[kk] & kResolved) ? v[kk] : g0 + kk;
+            const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
+            out[kk] = m ? (cur << 4) | m : (g0 + kk) << 4;   // << 4 drops the tag bit
         }
         uint32_t *dst = G.P + (size_t)by * bw + bxs;
-        if ((bw % 4u) == 0 && bxs + SEG <= bw) {
-            reinterpret_cast<uint4 *>(dst)[0] = make_uint4(out[0], out[1], out[2], out[3]);
-            reinterpret_cast<uint4 *>(dst)[1] = make_uint4(out[4], out[5], out[6], out[7]);
+        if ((bw % 4u) == 0 && bxs + LANE_BLOCKS <= bw) {
+            uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+            for (uint32_t q = 0; q < LANE_BLOCKS / 4; ++q) d4[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
         } else {
-            for (uint32_t k = 0; k < SEG; ++k)
-                if (bxs + k < bw) dst[k] = out[k];
+            for (uint32_t kk = 0; kk < LANE_BLOCKS && bxs + kk < bw; ++kk) dst[kk] = out[kk];
         }
     }
 }
@@ -347,25 +470,24 @@ __global__ __launch_bounds__(256) void bleq2d_flatten(Src src, uint32_t total_ti
 
 // ---- 4. final kernel -------------------------------------------------------------------
 
-// Eight labels of one pixel row starting at column x0 (x0 % 8 == 0): the widest
+// Four labels of one pixel row starting at column x0 (x0 % 4 == 0): the widest
 // stores the row's actual address alignment allows (16, 8 or 4 bytes), so
 // images of any width keep vector stores on most rows.
-__device__ __forceinline__ void store_row8(uint32_t *dst, const uint32_t v[8], uint32_t x0, uint32_t w)
+__device__ __forceinline__ void store_row4(uint32_t *dst, const uint32_t v[4], uint32_t x0, uint32_t w)
 {
     const uintptr_t a = reinterpret_cast<uintptr_t>(dst);
-    if (x0 + 8 <= w) {
+    if (x0 + 4 <= w) {
         if ((a & 15u) == 0) {
             store_nt4(dst, v[0], v[1], v[2], v[3]);
-            store_nt4(dst + 4, v[4], v[5], v[6], v[7]);
             return;
         }
         if ((a & 7u) == 0) {
-#pragma unroll
-            for (int i = 0; i < 8; i += 2) store_nt2(dst + i, v[i], v[i + 1]);
+            store_nt2(dst, v[0], v[1]);
+            store_nt2(dst + 2, v[2], v[3]);
             return;
         }
     }
-    for (uint32_t i = 0; i < 8 && x0 + i < w; ++i) dst[i] = v[i];
+    for (uint32_t i = 0; i < 4 && x0 + i < w; ++i) dst[i] = v[i];
 }
 
 template <class Src>
@@ -377,18 +499,37 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
     const uint32_t by = lw / G.xgroups;
     const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
     const uint32_t *__restrict__ P = G.P;
-    uint32_t roots = 0;
-    if (by < bh && bxs < bw) {
+    const bool inside = by < bh && bxs < bw;
+    uint32_t v[FINAL_BLOCKS] = {};
+    if (inside) {
         const uint32_t *src_w = P + (size_t)by * bw + bxs;
-        uint32_t v[FINAL_BLOCKS];
-        if ((reinterpret_cast<uintptr_t>(src_w) & 15u) == 0 && bxs + FINAL_BLOCKS <= bw) {
-            const uint4 q = *reinterpret_cast<const uint4 *>(src_w);
-            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        if ((reinterpret_cast<uintptr_t>(src_w) & 7u) == 0 && bxs + FINAL_BLOCKS <= bw) {
+            const uint2 q = *reinterpret_cast<const uint2 *>(src_w);
+            v[0] = q.x; v[1] = q.y;
         } else {
             for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = bxs + k < bw ? src_w[k] : 0u;
         }
+    }
+    // Neighbouring lanes mostly share their blocks' parent (a tile-local root):
+    // the first lane of each run of lanes with the same first parent chases
+    // it, the others take its root.
+    uint32_t key = 0;
+    bool has = false;
+#pragma unroll
+    for (uint32_t k = FINAL_BLOCKS; k-- > 0;)
+        if (v[k] & 15u) {
+            key = v[k] >> 4;
+            has = true;
+        }
+    uint32_t leader;
+    const bool lead = run_leader(key, 0u, has, leader);
+    uint32_t key_root = lead ? Forest::find_final(P, key) : 0u;
+    key_root = __shfl(key_root, (int)leader, 64);
+
+    uint32_t roots = 0;
+    if (inside) {
         uint32_t lab[FINAL_BLOCKS];
-        uint32_t prev_parent = 0xFFFFFFFFu, prev_root = 0;
+        uint32_t prev_parent = key, prev_root = key_root;
         const uint32_t b0 = by * bw + bxs;
         const bool ref = (G.flags & kRefLabels) != 0;
 #pragma unroll
@@ -416,8 +557,8 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
             bot[2 * k] = (m & 4u) ? lab[k] : 0u;
             bot[2 * k + 1] = (m & 8u) ? lab[k] : 0u;
         }
-        store_row8(row0, top, x0, w);
-        if (two_rows) store_row8(row1, bot, x0, w);
+        store_row4(row0, top, x0, w);
+        if (two_rows) store_row4(row1, bot, x0, w);
     }
     if (G.count) count_add(G.slots, roots);   // one image per workgroup: uniform branch
 }
@@ -434,7 +575,7 @@ __global__ __launch_bounds__(kCountSlots) void bleq2d_count(Src src)
 
 template <uint32_t TBW, uint32_t TBH>
 struct Variant {
-    static constexpr uint32_t THREADS = (TBW / SEG) * TBH;
+    static constexpr uint32_t THREADS = (TBW / LANE_BLOCKS) * TBH;
 
     template <class Src>
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
@@ -464,20 +605,20 @@ hipError_t dispatch(uint32_t tile_id, const Src &src, uint32_t total_tiles, uint
                     KernelTimer *kt, bool flatten)
 {
     switch (tile_id) {
-    case 1: return Variant<64, 16>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 2: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 3: return Variant<256, 8>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 4: return Variant<64, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 5: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 6: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
-    default: return Variant<128, 16>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 1: return Variant<64, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 2: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 3: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 4: return Variant<256, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 5: return Variant<512, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 6: return Variant<128, 128>::launch(src, total_tiles, total_final, s, kt, flatten);
+    default: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
     }
 }
 
 }  // namespace
 
-const TileShape2D kBleq2dTiles[kBleq2dTileCount] = {{128, 16}, {64, 16}, {128, 32}, {256, 8},
-                                                    {64, 32},  {128, 64}, {256, 32}};
+const TileShape2D kBleq2dTiles[kBleq2dTileCount] = {{128, 32}, {64, 64},  {128, 64}, {256, 32},
+                                                    {256, 64}, {512, 32}, {128, 128}};
 
 void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
 {

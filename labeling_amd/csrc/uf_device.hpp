@@ -106,10 +106,11 @@ template <int SHIFT>
 struct PackedForest {
     static constexpr uint32_t kLowMask = (1u << SHIFT) - 1u;
 
-    // Root of x with path halving (x -> grandparent, by atomicMin so a link
-    // never grows); the starting element is then pointed at the root, so the
-    // next find from it (typically another border union of the same tile-local
-    // root) takes one hop.
+    // Root of x by a read-only chase; the starting element is then pointed at
+    // the root, so the next find from it (typically another border union of
+    // the same tile-local root) takes one hop.  (Halving every visited entry
+    // costs a device-scope atomic per hop, and the entries near a large
+    // component's root are visited by thousands of finds at once.)
     __device__ static uint32_t find(uint32_t *w, uint32_t x)
     {
         const uint32_t x0 = x;
@@ -121,7 +122,6 @@ struct PackedForest {
             const uint32_t vp = dev_load(&w[p]);
             const uint32_t gp = vp >> SHIFT;
             if (gp == p) { root = p; break; }
-            dev_min(&w[x], (gp << SHIFT) | (v & kLowMask));
             x = gp;
             v = dev_load(&w[x]);
         }
