@@ -10,26 +10,28 @@
 // per pass; here the equivalences are resolved by union-find in four launches
 // and no host synchronisation:
 //
-//   1. tile    one workgroup per TBW x TBH-block tile: 16-byte row loads ->
-//              4-bit block masks; union-find of the tile's blocks in LDS;
-//              writes one packed word per block, parent << 4 | mask, whose
-//              parent is the tile-local root (global block index).
+//   1. tile    one workgroup per TBW x TBH-block tile: each lane turns two
+//              32-pixel row segments into bit masks; union-find over the
+//              runs of each 2-row band in LDS; writes one packed word per
+//              block, parent << 4 | mask, whose parent is the tile-local root
+//              (global block index).
 //   2. merge   one thread per block on a tile border: unions with each
 //              8-neighbour block in another tile that it touches (atomicMin
 //              links between tile-local roots).
-//   3. flatten one thread per tile-border block: the block and its tile-local
-//              root are pointed straight at the global root.
-//   4. final   one thread per 4 blocks: root of the block (<= 2 hops), writes
-//              the 8 labels of the 2x2 pixels of each block, 0 for background,
-//              and counts roots.
+//   3. flatten (optional, chosen by the autotuner) one thread per tile-border
+//              block: the block and its tile-local root are pointed straight
+//              at the global root.
+//   4. final   one thread per 2 blocks: root of the block, writes the labels
+//              of the 2x2 pixels of each block, 0 for background, and counts
+//              roots into 64 spread slots, summed by a one-wave count kernel.
 //
 // Every kernel reads its image geometry from a "source": One (a single image
 // passed by value) or Many (a device table of images; a workgroup finds its
 // image by binary search over the images' first tile / workgroup ids), so a
 // batch of thousands of images runs as the same four launches.
 //
-// HBM traffic per pixel: 1 B image + 1 B block words in (1), 1 B block words +
-// 4 B labels in (4), plus the border blocks in (2)-(3).
+// HBM traffic per pixel: 1 B image in + 1 B block words out in (1), 1 B block
+// words in + 4 B labels out in (4), plus the border blocks in (2)-(3).
 #include "bleq2d.hpp"
 #include "uf_device.hpp"
 
