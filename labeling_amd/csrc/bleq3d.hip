@@ -7,7 +7,8 @@
 // equivalent when a foreground voxel of one is 26-adjacent to a foreground voxel
 // of the other, and each component is labelled by its minimum block.  The
 // reference iterates whole-grid Scan/Analyze passes with a host round trip per
-// pass; here, as in 2D, four launches resolve everything with union-find:
+// pass; here three launches (and a one-wave count) resolve everything with
+// union-find:
 //
 //   1. tile    one workgroup (512 threads) per 64x8x8-block tile
 //              (128x16x16 voxels): four 16-byte row loads per thread -> 8-bit
@@ -15,9 +16,9 @@
 //              tile-local root (global block index) and the mask of every block.
 //   2. merge   one thread per block on a tile face: unions with each earlier
 //              26-neighbour block in another tile that it touches.
-//   3. flatten one thread per tile-face block: its tile-local root -> global root.
-//   4. final   one thread per 4 blocks: root (<= 2 hops), writes the 32 voxel
-//              labels of the 4 blocks, counts roots.
+//   3. final   one thread per 2 blocks: root of each block (neighbouring lanes
+//              share one chase per tile-local root), writes the 16 voxel
+//              labels of the 2 blocks, counts roots.
 //
 // Block mask bit of voxel (dx, dy, dz) in {0,1}^3 is dx + 2 dy + 4 dz.
 #include "lbl_internal.hpp"
@@ -33,7 +34,7 @@ constexpr uint32_t NT_TILE = SEGS * TY * TZ;     // 512
 constexpr uint32_t TILE_BLOCKS = TX * TY * TZ;   // 4096
 constexpr uint32_t FACE_Z = TX * TY, FACE_Y = TX * (TZ - 2), FACE_X = (TY - 2) * (TZ - 2);
 constexpr uint32_t JOBS = 2 * FACE_Z + 2 * FACE_Y + 2 * FACE_X;
-constexpr uint32_t FINAL_BLOCKS = 4;
+constexpr uint32_t FINAL_BLOCKS = 2;                // one 16-byte store per voxel row per thread
 constexpr uint32_t NT_FINAL = 256;
 
 using Forest = PackedForest<0>;
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict
     }
 }
 
-// ---- 2./3. tile-face kernels -----------------------------------------------------
+// ---- 2. tile-face merge -----------------------------------------------------
 
 __device__ __forceinline__ void face_block(uint32_t j, uint32_t &lx, uint32_t &ly, uint32_t &lz)
 {
@@ -294,15 +295,7 @@ __global__ __launch_bounds__(256) void bleq3d_merge(uint32_t *P, const uint8_t *
     }
 }
 
-__global__ __launch_bounds__(256) void bleq3d_flatten(uint32_t *P, const uint8_t *__restrict__ M, Grid3 g)
-{
-    uint32_t bx, by, bz, bx0, by0, bz0;
-    if (!job_block(g, blockIdx.x * blockDim.x + threadIdx.x, bx, by, bz, bx0, by0, bz0)) return;
-    const uint32_t b = (bz * g.bh + by) * g.bw + bx;
-    if (M[b]) Forest::compress(P, b);
-}
-
-// ---- 4. final kernel --------------------------------------------------------------
+// ---- 3. final kernel --------------------------------------------------------------
 
 template <bool VEC>
 __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
@@ -313,23 +306,42 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
     const uint32_t row = blockIdx.x / xgroups;           // block row index bz * bh + by
     const uint32_t by = row % bh, bz = row / bh;
     const uint32_t bxs = ((blockIdx.x % xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
-    uint32_t roots = 0;
-    if (bz < bd && bxs < bw) {
-        const size_t b0 = ((size_t)bz * bh + by) * bw + bxs;
-        uint32_t pv[FINAL_BLOCKS], mv[FINAL_BLOCKS];
+    const bool inside = bz < bd && bxs < bw;
+    const size_t b0 = ((size_t)bz * bh + by) * bw + bxs;
+    uint32_t pv[FINAL_BLOCKS] = {}, mv[FINAL_BLOCKS] = {};
+    if (inside) {
         if (VEC && bxs + FINAL_BLOCKS <= bw) {
-            const uint4 q = *reinterpret_cast<const uint4 *>(P + b0);
-            const uint32_t mm = *reinterpret_cast<const uint32_t *>(M + b0);
-            pv[0] = q.x; pv[1] = q.y; pv[2] = q.z; pv[3] = q.w;
-            for (int k = 0; k < (int)FINAL_BLOCKS; ++k) mv[k] = (mm >> (8 * k)) & 0xFFu;
+            const uint2 q = *reinterpret_cast<const uint2 *>(P + b0);
+            const uint32_t mm = *reinterpret_cast<const uint16_t *>(M + b0);
+            pv[0] = q.x; pv[1] = q.y;
+            mv[0] = mm & 0xFFu; mv[1] = mm >> 8;
         } else {
             for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
                 pv[k] = bxs + k < bw ? P[b0 + k] : 0u;
                 mv[k] = bxs + k < bw ? M[b0 + k] : 0u;
             }
         }
+    }
+    // Neighbouring lanes mostly share their blocks' parent (a tile-local root):
+    // the first lane of each run of lanes with the same first parent chases
+    // it, the others take its root.
+    uint32_t key = 0;
+    bool has = false;
+#pragma unroll
+    for (uint32_t k = FINAL_BLOCKS; k-- > 0;)
+        if (mv[k]) {
+            key = pv[k];
+            has = true;
+        }
+    uint32_t leader;
+    const bool lead = run_leader(key, 0u, has, leader);
+    uint32_t key_root = lead ? Forest::find_final(P, key) : 0u;
+    key_root = __shfl(key_root, (int)leader, 64);
+
+    uint32_t roots = 0;
+    if (inside) {
         uint32_t lab[FINAL_BLOCKS];
-        uint32_t prev_parent = 0xFFFFFFFFu, prev_root = 0;
+        uint32_t prev_parent = key, prev_root = key_root;
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
             lab[k] = 0;
@@ -356,7 +368,6 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
             uint32_t *dst = labels + ((size_t)z * h + y) * w + x0;
             if (VEC && x0 + 2 * FINAL_BLOCKS <= w) {
                 store_nt4(dst, r[0], r[1], r[2], r[3]);
-                store_nt4(dst + 4, r[4], r[5], r[6], r[7]);
             } else {
                 for (uint32_t i = 0; i < 2 * FINAL_BLOCKS && x0 + i < w; ++i) dst[i] = r[i];
             }
@@ -398,8 +409,6 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     if (kt) kt->mark(s, "bleq3d_tile");
     hipLaunchKernelGGL(bleq3d_merge, dim3(face_grid), dim3(256), 0, s, P, M, g);
     if (kt) kt->mark(s, "bleq3d_merge");
-    hipLaunchKernelGGL(bleq3d_flatten, dim3(face_grid), dim3(256), 0, s, P, M, g);
-    if (kt) kt->mark(s, "bleq3d_flatten");
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
     if (vec_out)
         hipLaunchKernelGGL(bleq3d_final<true>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd, bh,

@@ -154,19 +154,6 @@ struct PackedForest {
             a = old >> SHIFT;
         }
     }
-
-    // After all unions: point element x and its tile-local root straight at
-    // the final root (every write is an atomicMin towards that root).
-    __device__ static void compress(uint32_t *w, uint32_t x)
-    {
-        const uint32_t vx = dev_load(&w[x]);
-        const uint32_t r = vx >> SHIFT;
-        const uint32_t vr = dev_load(&w[r]);
-        if ((vr >> SHIFT) == r) return;          // already one hop from the root
-        const uint32_t root = find(w, r);
-        dev_min(&w[r], (root << SHIFT) | (vr & kLowMask));
-        if (r != x) dev_min(&w[x], (root << SHIFT) | (vx & kLowMask));
-    }
 };
 
 // ---- label forest: label = index + 1, 0 = background -----------------------
