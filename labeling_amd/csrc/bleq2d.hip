@@ -133,35 +133,7 @@ struct Many {
 // points both of its nodes at the root, which keeps later finds short), then
 // resolves its nodes' roots and writes the 16 block words of its segment.
 
-constexpr uint32_t LANE_BLOCKS = 16;          // blocks per lane (32 pixel columns)
-constexpr uint32_t kResolved = 0x80000000u;   // LDS tag: entry holds the global root
-
-// bit i set <=> byte i of w is non-zero (i = 0..3)
-__device__ __forceinline__ uint32_t nz_bits4(uint32_t w)
-{
-    const uint32_t nz = (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
-    return (nz * 0x00204081u) >> 28;
-}
-
-__device__ __forceinline__ uint32_t nz_bits16(uint4 q)
-{
-    return nz_bits4(q.x) | (nz_bits4(q.y) << 4) | (nz_bits4(q.z) << 8) | (nz_bits4(q.w) << 12);
-}
-
-__device__ __forceinline__ uint32_t hibit(uint32_t v) { return 31u - __builtin_clz(v); }   // v != 0
-__device__ __forceinline__ uint32_t upto(uint32_t x) { return (2u << x) - 1u; }           // bits 0..x
-
-// 32 pixels of one row as a bit mask (bit k = column x0 + k), zero past the width.
-__device__ __forceinline__ uint32_t row_bits32(const uint8_t *row, uint32_t x0, uint32_t w, bool vec)
-{
-    if (vec && x0 + 32 <= w) {
-        const uint4 *q = reinterpret_cast<const uint4 *>(row + x0);
-        return nz_bits16(q[0]) | (nz_bits16(q[1]) << 16);
-    }
-    uint32_t m = 0;
-    for (uint32_t k = 0; k < 32 && x0 + k < w; ++k) m |= (uint32_t)(row[x0 + k] != 0) << k;
-    return m;
-}
+constexpr uint32_t LANE_BLOCKS = kLaneBlocks;
 
 template <uint32_t TBW, uint32_t TBH, class Src>
 __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src)
@@ -194,20 +166,8 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
     const uint32_t st = o & ~(o << 1);               // first column of each segment-local run
     const uint32_t nb = r * TBW + j * LANE_BLOCKS;   // local index of the segment's first block
 
-    // head of the band run this segment's first run continues (if it does)
-    uint32_t first = nb;
-    {
-        const uint32_t lw = t & 63u, base = lw - j;   // lane of segment 0 of this band
-        const uint64_t nonfull = __ballot(o != 0xFFFFFFFFu);
-        const uint64_t below = nonfull & ((1ull << lw) - 1ull) & ~((1ull << base) - 1ull);
-        const uint32_t k = below ? 63u - (uint32_t)__builtin_clzll(below) : lw;
-        const uint32_t o_left = __shfl(o, (int)((lw + 63u) & 63u), 64);
-        const uint32_t o_k = __shfl(o, (int)k, 64);
-        if (j > 0 && (o & 1u) && (o_left >> 31)) {
-            const uint32_t sk = r * TBW + (k - base) * LANE_BLOCKS;   // segment k's first block
-            first = !below ? r * TBW : ((o_k >> 31) ? sk + (hibit(o_k & ~(o_k << 1)) >> 1) : sk + LANE_BLOCKS);
-        }
-    }
+    // node standing for the segment's first run (the band run's head if it continues from the left)
+    const uint32_t first = run_head(o, j, r * TBW);
 
     so[r + 1][j + 1] = o;
     sb[r + 1][j + 1] = b;
@@ -229,69 +189,23 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
 
     // unions along the vertical contacts with the band above
     {
-        const uint32_t ou = so[r][j + 1], bu = sb[r][j + 1], bul = sb[r][j], bur = sb[r][j + 2], oul = so[r][j];
-        const uint32_t stu = ou & ~(ou << 1);
-        const uint32_t nbu = nb - TBW;   // only used when a contact exists (r > 0)
-        const uint32_t su_l = oul ? hibit(oul & ~(oul << 1)) : 0u;
-        const uint32_t ul_node = su_l ? nbu - LANE_BLOCKS + (su_l >> 1) : sh[r][j];   // upper-left segment's last run
-        const uint32_t ur_node = sh[r][j + 2];                                         // upper-right segment's first run
-        const uint32_t up_first = sh[r][j + 1];
-        // straight contacts: the first column of each run of a & b_up, and of
-        // those only the first after a lower or upper run boundary (carry trick:
-        // adding the boundary bits to the complement runs each carry up to the
-        // next contact of its segment); diagonal contacts only where neither
-        // the straight contact nor the neighbouring top-row pixel exists
-        uint32_t e0, el, er;
-        {
-            const uint32_t v0 = a & bu, f0 = v0 & ~(v0 << 1), bnd = st | stu | 1u;
-            const uint32_t y = ~f0 & ~(bnd >> 1);
-            e0 = ((y + bnd) & ~y) & f0;
-            el = a & ((bu << 1) | (bul >> 31)) & ~bu & ~(a << 1);
-            er = a & ((bu >> 1) | (bur << 31)) & ~bu & ~(a >> 1);
-        }
-        uint32_t ua = 0, ub = 0, oa = 0, ob = 0, rest = e0 | el | er;
+        const uint32_t ou = so[r][j + 1];
+        const SegNodes up{ou & ~(ou << 1), nb - TBW, sh[r][j + 1], last_run_node(so[r][j], nb - TBW - LANE_BLOCKS, sh[r][j]),
+                          sh[r][j + 2]};   // only used when a contact exists (r > 0)
+        SegContacts cs(a, sb[r][j + 1], sb[r][j], sb[r][j + 2], st, up.st);
+        uint32_t ua = 0, ub = 0, oa = 0, ob = 0;
         bool busy = false;
         for (;;) {
             if (!busy) {
-                if (!rest) break;
-                // next contact: upper column x-1 before x before x+1
-                const uint32_t x = __builtin_ctz(rest), bit = 1u << x;
-                const bool tl = (el & bit) != 0, t0 = (e0 & bit) != 0;
-                el ^= tl ? bit : 0u;
-                e0 ^= (!tl && t0) ? bit : 0u;
-                er ^= (!tl && !t0) ? bit : 0u;
-                rest = e0 | el | er;
-                const int p = (int)x + (tl ? -1 : (t0 ? 0 : 1));
+                if (!cs.any()) break;
+                uint32_t x;
+                const int p = cs.pop(x);
                 const uint32_t s = hibit(st & upto(x));
-                ua = s ? nb + (s >> 1) : first;
-                if (p < 0) {
-                    ub = ul_node;
-                } else if (p > 31) {
-                    ub = ur_node;
-                } else {
-                    const uint32_t su = hibit(stu & upto((uint32_t)p));
-                    ub = su ? nbu + (su >> 1) : up_first;
-                }
-                oa = ua;
-                ob = ub;
+                oa = ua = s ? nb + (s >> 1) : first;
+                ob = ub = up.node(p);
                 busy = true;
             }
-            const uint32_t pa = lds_load(&par[ua]), pb = lds_load(&par[ub]);
-            if (pa != ua || pb != ub) {   // not both roots yet: one hop each
-                ua = pa;
-                ub = pb;
-                continue;
-            }
-            if (ua != ub) {   // link the larger root under the smaller
-                const uint32_t hi = ua > ub ? ua : ub, lo = ua ^ ub ^ hi;
-                const uint32_t old = lds_min(&par[hi], lo);
-                if (old != hi) {   // hi stopped being a root meanwhile
-                    ua = old;
-                    ub = lo;
-                    continue;
-                }
-                ua = lo;
-            }
+            if (!lds_union_step(par, ua, ub)) continue;
             // both original nodes now point straight at the set's root (or lower)
             lds_min(&par[oa], ua);
             lds_min(&par[ob], ua);
@@ -319,28 +233,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
     for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk)
         if ((st & (3u << (2 * kk))) && v[kk] != nb + kk) pend |= 1u << kk;
     if (pend) {
-        uint32_t cur = 0, orig = 0;
-        bool busy = false;
-        for (;;) {
-            if (!busy) {
-                if (!pend) break;
-                orig = cur = nb + __builtin_ctz(pend);
-                pend &= pend - 1;
-                busy = true;
-            }
-            const uint32_t p = lds_load(&par[cur]);
-            uint32_t g;
-            if (p & kResolved) {
-                g = p;
-            } else if (p == cur) {
-                g = kResolved | ((by0 + (cur >> LOG_TBW)) * bw + bx0 + (cur & (TBW - 1)));
-            } else {
-                cur = p;
-                continue;
-            }
-            lds_store(&par[orig], g);
-            busy = false;
-        }
+        resolve_nodes(par, nb, pend, [&](uint32_t l) { return (by0 + (l >> LOG_TBW)) * bw + bx0 + (l & (TBW - 1)); });
         load_nodes();
     }
 

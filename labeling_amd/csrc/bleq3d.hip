@@ -10,10 +10,11 @@
 // pass; here three launches (and a one-wave count) resolve everything with
 // union-find:
 //
-//   1. tile    one workgroup (512 threads) per 64x8x8-block tile
-//              (128x16x16 voxels): four 16-byte row loads per thread -> 8-bit
-//              block masks; union-find of the tile's blocks in LDS; writes the
-//              tile-local root (global block index) and the mask of every block.
+//   1. tile    one workgroup (512 threads) per 32x16x16-block tile
+//              (64x32x32 voxels): each lane turns four 32-voxel rows into bit
+//              masks; union-find over the runs of each 2x2 bar of voxel rows
+//              in LDS; writes the tile-local root (global block index) and the
+//              mask of every block.
 //   2. merge   one thread per block on a tile face: unions with each earlier
 //              26-neighbour block in another tile that it touches.
 //   3. final   one thread per 2 blocks: root of each block (neighbouring lanes
@@ -27,11 +28,7 @@
 namespace lbl {
 namespace {
 
-constexpr uint32_t TX = 64, TY = 8, TZ = 8;       // tile, blocks
-constexpr uint32_t SEG = 8;                      // blocks per thread along x
-constexpr uint32_t SEGS = TX / SEG;              // 8 threads per block row
-constexpr uint32_t NT_TILE = SEGS * TY * TZ;     // 512
-constexpr uint32_t TILE_BLOCKS = TX * TY * TZ;   // 4096
+constexpr uint32_t TX = 32, TY = 16, TZ = 16;     // tile, blocks (64 x 32 x 32 voxels)
 constexpr uint32_t FACE_Z = TX * TY, FACE_Y = TX * (TZ - 2), FACE_X = (TY - 2) * (TZ - 2);
 constexpr uint32_t JOBS = 2 * FACE_Z + 2 * FACE_Y + 2 * FACE_X;
 constexpr uint32_t FINAL_BLOCKS = 2;                // one 16-byte store per voxel row per thread
@@ -71,160 +68,193 @@ static_assert(side_mask(0, -1, 0, false) == 0x33 && side_mask(0, -1, 0, true) ==
 static_assert(side_mask(0, 0, -1, false) == 0x0F && side_mask(0, 0, -1, true) == 0xF0, "z faces");
 static_assert(side_mask(-1, -1, -1, false) == 0x01 && side_mask(-1, -1, -1, true) == 0x80, "corner");
 
-// Two 16-voxel rows of one z-plane pair -> masks of 8 blocks (8 bits each),
-// split into the low (blocks 0-3) and high (blocks 4-7) 32-bit halves.
-__device__ __forceinline__ void block_masks(const uint4 &r00, const uint4 &r01, const uint4 &r10, const uint4 &r11,
-                                            uint32_t &lo, uint32_t &hi)
-{
-    const uint32_t a[4] = {r00.x, r00.y, r00.z, r00.w}, b[4] = {r01.x, r01.y, r01.z, r01.w};
-    const uint32_t c[4] = {r10.x, r10.y, r10.z, r10.w}, d[4] = {r11.x, r11.y, r11.z, r11.w};
-    uint32_t out[2] = {0, 0};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint32_t z0 = quad_masks(a[q], b[q]);   // nibble per block, plane z0
-        const uint32_t z1 = quad_masks(c[q], d[q]);   // plane z1
-        const uint32_t m0 = (z0 & 15u) | ((z1 & 15u) << 4);
-        const uint32_t m1 = (z0 >> 4) | (z1 & 0xF0u);
-        out[q >> 1] |= (m0 | (m1 << 8)) << (16 * (q & 1));
-    }
-    lo = out[0];
-    hi = out[1];
-}
-
-__device__ __forceinline__ uint4 load_row16(const uint8_t *row, uint32_t x0, uint32_t w, bool vec)
-{
-    if (vec && x0 + 16 <= w) return *reinterpret_cast<const uint4 *>(row + x0);
-    uint32_t q[4] = {0, 0, 0, 0};
-    for (uint32_t k = 0; k < 16; ++k)
-        if (x0 + k < w && row[x0 + k]) q[k >> 2] |= 0xFFu << (8 * (k & 3));
-    return make_uint4(q[0], q[1], q[2], q[3]);
-}
-
-__device__ __forceinline__ uint32_t mask_at(uint32_t lo, uint32_t hi, int k)
-{
-    return ((k < 4 ? lo : hi) >> (8 * (k & 3))) & 0xFFu;
-}
-
 // ---- 1. tile kernel -------------------------------------------------------------
+//
+// A tile is TBW x TBY x TBZ blocks.  A "bar" is one row of blocks along x; one
+// lane holds a 32-voxel segment of a bar (16 blocks) as the masks of its four
+// voxel rows: a = (y0, z0), b = (y1, z0), c = (y0, z1), e = (y1, z1).  All
+// lanes of a bar are consecutive lanes of one wave.  Within a bar,
+// 26-connectivity is exactly "same run of set bits of o = a | b | c | e" (two
+// voxels in adjacent columns of a 2 x 2 bar always touch), so, as in 2D, the
+// union-find nodes are the segment-local runs, a run continuing the left
+// segment stands for the head of the whole bar run, and contacts with the four
+// earlier neighbour bars are found with bit operations on facing rows:
+//   (y-1):      a | c  against the neighbour's b | e
+//   (z-1):      a | b  against the neighbour's c | e
+//   (y-1, z-1): a      against the neighbour's e
+//   (y+1, z-1): b      against the neighbour's c
+// (the later neighbours meet this bar as one of their earlier ones).  Each lane
+// feeds its contacts through one union loop, then resolves its nodes' roots and
+// writes its 16 block words and masks.
 
-template <bool VEC>
-__global__ __launch_bounds__(NT_TILE) void bleq3d_tile(const uint8_t *__restrict__ vox, uint32_t d, uint32_t h,
-                                                        uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
-                                                        uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P,
-                                                        uint8_t *__restrict__ M, uint32_t *count, uint32_t *slots)
+template <uint32_t TBW, uint32_t TBY, uint32_t TBZ>
+struct Tile3 {
+    static constexpr uint32_t LPB = TBW / kLaneBlocks;          // lanes per bar
+    static constexpr uint32_t THREADS = LPB * TBY * TBZ;
+    static constexpr uint32_t BLOCKS = TBW * TBY * TBZ;
+    static_assert((TBW & (TBW - 1)) == 0 && (TBY & (TBY - 1)) == 0 && LPB >= 1 && LPB <= 64, "tile shape");
+    static constexpr uint32_t LOG_TBW = __builtin_ctz(TBW), LOG_TBY = __builtin_ctz(TBY);
+    // per-bar LDS tables: [z + 1][y + 1][segment + 1], zero guard entries around
+    static constexpr uint32_t GZ = TBZ + 1, GY = TBY + 2, GX = LPB + 2, ENTRIES = GZ * GY * GX;
+    static __device__ __forceinline__ uint32_t at(uint32_t gz, uint32_t gy, uint32_t gx) { return (gz * GY + gy) * GX + gx; }
+};
+
+template <uint32_t TBW, uint32_t TBY, uint32_t TBZ, bool VEC>
+__global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
+    const uint8_t *__restrict__ vox, uint32_t d, uint32_t h, uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
+    uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P, uint8_t *__restrict__ M, uint32_t *count,
+    uint32_t *slots)
 {
-    __shared__ uint32_t par[TILE_BLOCKS];
-    const SwizzleMap<SEG, NT_TILE> at;   // conflict-free placement of the forest
-    __shared__ uint2 mw[TZ][TY][SEGS + 2];   // guard column on each side
+    using T = Tile3<TBW, TBY, TBZ>;
+    constexpr uint32_t LPB = T::LPB;
+    __shared__ uint32_t par[T::BLOCKS];   // node (run start block) -> parent, local indices
+    __shared__ uint32_t so[T::ENTRIES];   // o = a | b | c | e
+    __shared__ uint32_t sbe[T::ENTRIES];  // b | e (the y1 rows)
+    __shared__ uint32_t sc[T::ENTRIES];   // c
+    __shared__ uint32_t se[T::ENTRIES];   // e
+    __shared__ uint32_t sh[T::ENTRIES];   // node standing for the segment's first run
 
-    const uint32_t t = threadIdx.x, s = t % SEGS, ly = (t / SEGS) % TY, lz = t / (SEGS * TY);
+    const uint32_t t = threadIdx.x, j = t % LPB, bar = t / LPB, ly = bar % TBY, lz = bar / TBY;
     const uint32_t tx = blockIdx.x % tiles_x, ty = (blockIdx.x / tiles_x) % tiles_y, tz = blockIdx.x / (tiles_x * tiles_y);
-    const uint32_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * TZ;
-    const uint32_t bxs = bx0 + s * SEG, by = by0 + ly, bz = bz0 + lz;
+    const uint32_t bx0 = tx * TBW, by0 = ty * TBY, bz0 = tz * TBZ;
+    const uint32_t bxs = bx0 + j * kLaneBlocks, by = by0 + ly, bz = bz0 + lz;
     if (blockIdx.x == 0 && count) count_clear(slots);
 
-    uint32_t lo = 0, hi = 0;
+    uint32_t a = 0, b = 0, c = 0, e = 0;
     if (bz < bd && by < bh && bxs < bw) {
         const uint32_t x0 = 2 * bxs, y0 = 2 * by, z0 = 2 * bz;
         const bool y1 = y0 + 1 < h, z1 = z0 + 1 < d;
-        const uint8_t *p00 = vox + ((size_t)z0 * h + y0) * w;
-        const uint4 zero = make_uint4(0, 0, 0, 0);
-        const uint4 r00 = load_row16(p00, x0, w, VEC);
-        const uint4 r01 = y1 ? load_row16(p00 + w, x0, w, VEC) : zero;
-        const uint4 r10 = z1 ? load_row16(p00 + (size_t)h * w, x0, w, VEC) : zero;
-        const uint4 r11 = (z1 && y1) ? load_row16(p00 + (size_t)h * w + w, x0, w, VEC) : zero;
-        block_masks(r00, r01, r10, r11, lo, hi);
+        const uint8_t *r00 = vox + ((size_t)z0 * h + y0) * w;
+        a = row_bits32(r00, x0, w, VEC);
+        if (y1) b = row_bits32(r00 + w, x0, w, VEC);
+        if (z1) c = row_bits32(r00 + (size_t)h * w, x0, w, VEC);
+        if (z1 && y1) e = row_bits32(r00 + (size_t)h * w + w, x0, w, VEC);
     }
-    mw[lz][ly][s + 1] = make_uint2(lo, hi);
-    if (s == 0) {
-        mw[lz][ly][0] = make_uint2(0, 0);
-        mw[lz][ly][SEGS + 1] = make_uint2(0, 0);
-    }
+    const uint32_t o = a | b | c | e;
+    const uint32_t st = o & ~(o << 1);                 // first column of each segment-local run
+    const uint32_t bar0 = bar * TBW, nb = bar0 + j * kLaneBlocks;
+    const uint32_t first = run_head(o, j, bar0);       // node standing for the segment's first run
 
-    const uint32_t li0 = (lz * TY + ly) * TX + s * SEG;
+    // guard entries (outside the tile) read as empty segments
+    for (uint32_t i = t; i < T::ENTRIES; i += T::THREADS) {
+        const uint32_t gx = i % T::GX, gy = (i / T::GX) % T::GY, gz = i / (T::GX * T::GY);
+        if (gz == 0 || gy == 0 || gy == TBY + 1 || gx == 0 || gx == LPB + 1) so[i] = sbe[i] = sc[i] = se[i] = 0;
+    }
+    const uint32_t me = T::at(lz + 1, ly + 1, j + 1);
+    so[me] = o;
+    sbe[me] = b | e;
+    sc[me] = c;
+    se[me] = e;
+    sh[me] = first;
     {
-        uint32_t head = li0, prev = 0;
+        uint4 *p4 = reinterpret_cast<uint4 *>(&par[nb]);
+        p4[0] = make_uint4(first, nb + 1, nb + 2, nb + 3);
 #pragma unroll
-        for (int k = 0; k < (int)SEG; ++k) {
-            const uint32_t m = mask_at(lo, hi, k), li = li0 + k;
-            if (m && adjacent(m, prev, -1, 0, 0)) {
-                par[at(li)] = head;
+        for (uint32_t q = 1; q < kLaneBlocks / 4; ++q) p4[q] = make_uint4(nb + 4 * q, nb + 4 * q + 1, nb + 4 * q + 2, nb + 4 * q + 3);
+    }
+    __syncthreads();
+
+    // unions along the contacts with the four earlier neighbour bars, one direction after the other
+    {
+        // direction q: neighbour bar offset (dy, dz) and the facing rows
+        const auto direction = [&](uint32_t q, SegContacts &cs, SegNodes &nn) {
+            const int dy = q == 1 ? 0 : (q == 3 ? 1 : -1), dz = q == 0 ? 0 : -1;
+            const uint32_t gz = lz + 1 + dz, gy = ly + 1 + dy;
+            const uint32_t i0 = T::at(gz, gy, j), i1 = i0 + 1, i2 = i0 + 2;   // left, same, right segment
+            uint32_t A, B, Bl, Br;
+            if (q == 0) {
+                A = a | c; B = sbe[i1]; Bl = sbe[i0]; Br = sbe[i2];
+            } else if (q == 1) {
+                A = a | b; B = sc[i1] | se[i1]; Bl = sc[i0] | se[i0]; Br = sc[i2] | se[i2];
+            } else if (q == 2) {
+                A = a; B = se[i1]; Bl = se[i0]; Br = se[i2];
             } else {
-                par[at(li)] = li;
-                head = li;
+                A = b; B = sc[i1]; Bl = sc[i0]; Br = sc[i2];
             }
-            prev = m;
+            const uint32_t on = so[i1];
+            const uint32_t nbar0 = (uint32_t)((int)bar + dy + dz * (int)TBY) * TBW;   // only used on a contact
+            nn.st = on & ~(on << 1);
+            nn.base = nbar0 + j * kLaneBlocks;
+            nn.first = sh[i1];
+            nn.left = last_run_node(so[i0], nn.base - kLaneBlocks, sh[i0]);
+            nn.right = sh[i2];
+            cs = SegContacts(A, B, Bl, Br, st, nn.st);
+        };
+        SegContacts cs(0, 0, 0, 0, 0, 0);
+        SegNodes nn{};
+        uint32_t q = 0;
+        direction(0, cs, nn);
+        uint32_t ua = 0, ub = 0, oa = 0, ob = 0;
+        bool busy = false;
+        for (;;) {
+            if (!busy) {
+                while (!cs.any() && q < 3) direction(++q, cs, nn);
+                if (!cs.any()) break;
+                uint32_t x;
+                const int p = cs.pop(x);
+                const uint32_t s = hibit(st & upto(x));
+                oa = ua = s ? nb + (s >> 1) : first;
+                ob = ub = nn.node(p);
+                busy = true;
+            }
+            if (!lds_union_step(par, ua, ub)) continue;
+            // both original nodes now point straight at the set's root (or lower)
+            lds_min(&par[oa], ua);
+            lds_min(&par[ob], ua);
+            busy = false;
         }
     }
     __syncthreads();
 
-    if (s > 0) {
-        const uint32_t m0 = mask_at(lo, hi, 0);
-        const uint32_t left = mw[lz][ly][s].y >> 24;
-        if (adjacent(m0, left, -1, 0, 0)) lds_union(par, li0, li0 - 1, at);
-    }
-    // earlier neighbour rows inside the tile: (dy, dz) = (-1,0), (-1,-1), (0,-1), (+1,-1)
-    const int rdy[4] = {-1, -1, 0, 1}, rdz[4] = {0, -1, -1, -1};
+    // Roots of this lane's nodes (self-roots: global index g0 + block; the
+    // others chased and replaced by the tagged global root).
+    const uint32_t g0 = (bz * bh + by) * bw + bxs;   // global index of local block nb
+    uint32_t v[kLaneBlocks];
+    const auto load_nodes = [&]() {
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(&par[nb]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int ny = (int)ly + rdy[q], nz = (int)lz + rdz[q];
-        if (ny < 0 || ny >= (int)TY || nz < 0) continue;
-        const uint2 up = mw[nz][ny][s + 1];
-        const uint32_t up_l = mw[nz][ny][s].y >> 24;
-        const uint32_t up_r = mw[nz][ny][s + 2].x & 0xFFu;
-        const int ubase = (int)(((uint32_t)nz * TY + (uint32_t)ny) * TX + s * SEG);
-        int last = -3;
-        uint32_t prev = 0;
-        for (int k = 0; k < (int)SEG; ++k) {
-            const uint32_t m = mask_at(lo, hi, k);
-            if (!m) {
-                prev = 0;
-                last = -3;
-                continue;
-            }
-            if (!adjacent(m, prev, -1, 0, 0)) last = -3;
-            prev = m;
-            for (int c = k - 1; c <= k + 1; ++c) {
-                const uint32_t um = c < 0 ? up_l : (c >= (int)SEG ? up_r : mask_at(up.x, up.y, c));
-                if (!adjacent(m, um, c - k, rdy[q], rdz[q]) || c == last) continue;
-                if (c == last + 1) {
-                    const uint32_t lm = last < 0 ? up_l : mask_at(up.x, up.y, last);
-                    if (adjacent(um, lm, -1, 0, 0)) {
-                        last = c;
-                        continue;
-                    }
-                }
-                lds_union(par, li0 + k, (uint32_t)(ubase + c), at);
-                last = c;
-            }
+        for (uint32_t q = 0; q < kLaneBlocks / 4; ++q) {
+            const uint4 x = p4[q];
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
         }
+    };
+    load_nodes();
+    uint32_t pend = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kLaneBlocks; ++k)
+        if ((st & (3u << (2 * k))) && v[k] != nb + k) pend |= 1u << k;
+    if (pend) {
+        resolve_nodes(par, nb, pend, [&](uint32_t l) {
+            const uint32_t q = l >> T::LOG_TBW;
+            return ((bz0 + (q >> T::LOG_TBY)) * bh + by0 + (q & (TBY - 1))) * bw + bx0 + (l & (TBW - 1));
+        });
+        load_nodes();
     }
-    __syncthreads();
 
+    // block words (root, own index for background) and masks (bit dx + 2 dy + 4 dz)
     if (bz < bd && by < bh && bxs < bw) {
-        uint32_t out[SEG];
-        uint32_t mk[2] = {lo, hi};
+        uint32_t out[kLaneBlocks], mk[kLaneBlocks / 4] = {0, 0, 0, 0}, cur = 0;
 #pragma unroll
-        for (int k = 0; k < (int)SEG; ++k) {
-            const uint32_t m = mask_at(lo, hi, k), li = li0 + k;
-            const uint32_t root = m ? lds_find(par, li, at) : li;
-            const uint32_t rx = root % TX, ry = (root / TX) % TY, rz = root / (TX * TY);
-            out[k] = ((bz0 + rz) * bh + by0 + ry) * bw + bx0 + rx;
+        for (uint32_t k = 0; k < kLaneBlocks; ++k) {
+            if (st & (3u << (2 * k))) cur = (v[k] & kResolved) ? (v[k] & ~kResolved) : g0 + k;
+            const uint32_t m = ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
+                               (((e >> (2 * k)) & 3u) << 6);
+            out[k] = m ? cur : g0 + k;
+            mk[k >> 2] |= m << (8 * (k & 3));
         }
         const size_t row = ((size_t)bz * bh + by) * bw + bxs;
         uint32_t *dst = P + row;
         uint8_t *mdst = M + row;
-        if ((bw % 4u) == 0 && bxs + SEG <= bw) {
-            reinterpret_cast<uint4 *>(dst)[0] = make_uint4(out[0], out[1], out[2], out[3]);
-            reinterpret_cast<uint4 *>(dst)[1] = make_uint4(out[4], out[5], out[6], out[7]);
-            *reinterpret_cast<uint32_t *>(mdst) = mk[0];
-            *reinterpret_cast<uint32_t *>(mdst + 4) = mk[1];
+        if ((bw % kLaneBlocks) == 0 && bxs + kLaneBlocks <= bw) {
+            uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+            for (uint32_t q = 0; q < kLaneBlocks / 4; ++q) d4[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+            *reinterpret_cast<uint4 *>(mdst) = make_uint4(mk[0], mk[1], mk[2], mk[3]);
         } else {
-            for (int k = 0; k < (int)SEG; ++k)
-                if (bxs + k < bw) {
-                    dst[k] = out[k];
-                    mdst[k] = (uint8_t)mask_at(lo, hi, k);
-                }
+            for (uint32_t k = 0; k < kLaneBlocks && bxs + k < bw; ++k) {
+                dst[k] = out[k];
+                mdst[k] = (uint8_t)(mk[k >> 2] >> (8 * (k & 3)));
+            }
         }
     }
 }
@@ -275,24 +305,39 @@ __device__ __forceinline__ bool job_block(const Grid3 &g, uint32_t gid, uint32_t
     return bx < g.bw && by < g.bh && bz < g.bd;
 }
 
+// One thread per tile-face block: unions with its earlier 26-neighbours in
+// other tiles, between tile-local roots.  Distinct root pairs only; the first
+// pair is skipped by every lane whose left neighbour lane holds the same one
+// (consecutive face blocks along x usually repeat it).
 __global__ __launch_bounds__(256) void bleq3d_merge(uint32_t *P, const uint8_t *__restrict__ M, Grid3 g)
 {
-    uint32_t bx, by, bz, bx0, by0, bz0;
-    if (!job_block(g, blockIdx.x * blockDim.x + threadIdx.x, bx, by, bz, bx0, by0, bz0)) return;
+    uint32_t bx = 0, by = 0, bz = 0, bx0 = 0, by0 = 0, bz0 = 0;
+    const bool inside = job_block(g, blockIdx.x * blockDim.x + threadIdx.x, bx, by, bz, bx0, by0, bz0);
     const uint32_t b = (bz * g.bh + by) * g.bw + bx;
-    const uint32_t m = M[b];
-    if (!m) return;
+    const uint32_t m = inside ? M[b] : 0u;
+    const uint32_t rb = m ? dev_load(&P[b]) : 0u;   // tile-local root
     // the 13 earlier neighbours: dz = -1 (9), dz = 0 & dy = -1 (3), dz = dy = 0 & dx = -1
-    for (int e = 0; e < 13; ++e) {
-        const int dz = e < 9 ? -1 : 0;
-        const int dy = e < 9 ? e / 3 - 1 : (e < 12 ? -1 : 0);
-        const int dx = e < 9 ? e % 3 - 1 : (e < 12 ? e - 10 : -1);
-        const uint32_t nx = bx + dx, ny = by + dy, nz = bz + dz;
-        if (nx >= g.bw || ny >= g.bh || nz >= g.bd) continue;
-        if (nx - bx0 < TX && ny - by0 < TY && nz - bz0 < TZ) continue;
-        const uint32_t n = (nz * g.bh + ny) * g.bw + nx;
-        if (adjacent(m, M[n], dx, dy, dz)) Forest::unite(P, P[b], P[n]);   // tile-local roots
+    uint32_t pr[13], np = 0;
+    if (m) {
+        for (int e = 0; e < 13; ++e) {
+            const int dz = e < 9 ? -1 : 0;
+            const int dy = e < 9 ? e / 3 - 1 : (e < 12 ? -1 : 0);
+            const int dx = e < 9 ? e % 3 - 1 : (e < 12 ? e - 10 : -1);
+            const uint32_t nx = bx + dx, ny = by + dy, nz = bz + dz;
+            if (nx >= g.bw || ny >= g.bh || nz >= g.bd) continue;
+            if (nx - bx0 < TX && ny - by0 < TY && nz - bz0 < TZ) continue;
+            const uint32_t n = (nz * g.bh + ny) * g.bw + nx;
+            if (!adjacent(m, M[n], dx, dy, dz)) continue;
+            const uint32_t rn = dev_load(&P[n]);
+            bool seen = rn == rb;
+            for (uint32_t i = 0; i < np; ++i) seen = seen || pr[i] == rn;
+            if (!seen) pr[np++] = rn;
+        }
     }
+    const uint64_t key = np ? ((uint64_t)rb << 32) | pr[0] : 0;
+    uint32_t leader;
+    const bool lead = run_leader(key, 0u, np != 0, leader);
+    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) Forest::unite(P, rb, pr[i]);
 }
 
 // ---- 3. final kernel --------------------------------------------------------------
@@ -400,12 +445,13 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     const uint32_t xgroups = (bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
 
     if (kt) kt->begin(s);
+    constexpr uint32_t NT_TILE = Tile3<TX, TY, TZ>::THREADS;
     if (vec_in)
-        hipLaunchKernelGGL(bleq3d_tile<true>, dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h, im.w, bd, bh,
-                           bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
+        hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
+                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
     else
-        hipLaunchKernelGGL(bleq3d_tile<false>, dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h, im.w, bd, bh,
-                           bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
+        hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
+                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_tile");
     hipLaunchKernelGGL(bleq3d_merge, dim3(face_grid), dim3(256), 0, s, P, M, g);
     if (kt) kt->mark(s, "bleq3d_merge");

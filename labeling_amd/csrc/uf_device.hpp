@@ -284,6 +284,180 @@ __device__ __forceinline__ void count_clear(uint32_t *slots)
     if (threadIdx.x < kCountSlots) slots[threadIdx.x * kCountStride] = 0;
 }
 
+// ---- bit masks of pixel rows ------------------------------------------------
+
+// bit i set <=> byte i of w is non-zero (i = 0..3)
+__device__ __forceinline__ uint32_t nz_bits4(uint32_t w)
+{
+    const uint32_t nz = (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+    return (nz * 0x00204081u) >> 28;
+}
+
+__device__ __forceinline__ uint32_t nz_bits16(uint4 q)
+{
+    return nz_bits4(q.x) | (nz_bits4(q.y) << 4) | (nz_bits4(q.z) << 8) | (nz_bits4(q.w) << 12);
+}
+
+__device__ __forceinline__ uint32_t hibit(uint32_t v) { return 31u - __builtin_clz(v); }   // v != 0
+__device__ __forceinline__ uint32_t upto(uint32_t x) { return (2u << x) - 1u; }           // bits 0..x
+
+// 32 pixels of one row as a bit mask (bit k = column x0 + k), zero past the width.
+__device__ __forceinline__ uint32_t row_bits32(const uint8_t *row, uint32_t x0, uint32_t w, bool vec)
+{
+    if (vec && x0 + 32 <= w) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(row + x0);
+        return nz_bits16(q[0]) | (nz_bits16(q[1]) << 16);
+    }
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < 32 && x0 + k < w; ++k) m |= (uint32_t)(row[x0 + k] != 0) << k;
+    return m;
+}
+
+// ---- run forests of bit-mask segments ------------------------------------------
+// The band / bar tile kernels give each lane a 32-pixel segment (16 blocks) of
+// one row of blocks; all lanes of a row are consecutive lanes of one wave.  A
+// segment-local run of set bits is a node named by the block of its first
+// column; the forest entry of a node is par[node], parent <= node.
+
+constexpr uint32_t kLaneBlocks = 16;          // blocks per lane (32 pixel columns)
+constexpr uint32_t kResolved = 0x80000000u;   // forest tag: entry holds the global root
+
+// Node standing for the first run of segment j of a row whose first block is
+// row0 (o: the segment's run mask): the segment's own first block, or, when
+// the run continues from the segment to the left, the first block of the
+// whole run -- found with a ballot over the row's lanes (the nearest segment
+// to the left that is not all set).  All lanes of the wave must call it.
+__device__ __forceinline__ uint32_t run_head(uint32_t o, uint32_t j, uint32_t row0)
+{
+    const uint32_t lw = threadIdx.x & 63u, base = lw - j;   // lane of segment 0 of this row
+    const uint64_t nonfull = __ballot(o != 0xFFFFFFFFu);
+    const uint64_t below = nonfull & ((1ull << lw) - 1ull) & ~((1ull << base) - 1ull);
+    const uint32_t k = below ? 63u - (uint32_t)__builtin_clzll(below) : lw;
+    const uint32_t o_left = __shfl(o, (int)((lw + 63u) & 63u), 64);
+    const uint32_t o_k = __shfl(o, (int)k, 64);
+    if (!(j > 0 && (o & 1u) && (o_left >> 31))) return row0 + j * kLaneBlocks;
+    const uint32_t sk = row0 + (k - base) * kLaneBlocks;   // segment k's first block
+    return !below ? row0 : ((o_k >> 31) ? sk + (hibit(o_k & ~(o_k << 1)) >> 1) : sk + kLaneBlocks);
+}
+
+// One step of uniting the sets of ua and ub: while either is not a root both
+// hop once; two different roots are linked, the larger under the smaller (a
+// failed link continues from the parent that displaced it).  Returns true once
+// united, with ua the root of the union (or an ancestor of it).
+__device__ __forceinline__ bool lds_union_step(uint32_t *par, uint32_t &ua, uint32_t &ub)
+{
+    const uint32_t pa = lds_load(&par[ua]), pb = lds_load(&par[ub]);
+    if (pa != ua || pb != ub) {
+        ua = pa;
+        ub = pb;
+        return false;
+    }
+    if (ua == ub) return true;
+    const uint32_t hi = ua > ub ? ua : ub, lo = ua ^ ub ^ hi;
+    const uint32_t old = lds_min(&par[hi], lo);
+    if (old != hi) {
+        ua = old;
+        ub = lo;
+        return false;
+    }
+    ua = lo;
+    return true;
+}
+
+// Contacts between a segment's facing row A and the facing row B of a
+// neighbouring row of segments, same columns (Bl / Br: that row's segments to
+// the left / right), where facing pixels touch when their columns differ by at
+// most one:
+//   e0      straight contacts: the first column of each run of A & B, and of
+//           those only the first after a run start of either row (st, stn;
+//           carry trick: adding the boundary bits to the complement runs each
+//           carry up to the next contact of its segment);
+//   el, er  diagonal contacts (B at x-1 / x+1), only where neither the
+//           straight contact nor the neighbouring A pixel exists (otherwise
+//           they repeat a straight contact's pair of runs).
+// Popped in column order (neighbour column x-1 before x before x+1), the pairs
+// of runs they join are monotone in both rows, so equal pairs are adjacent.
+struct SegContacts {
+    uint32_t e0, el, er;
+    __device__ __forceinline__ SegContacts(uint32_t A, uint32_t B, uint32_t Bl, uint32_t Br, uint32_t st,
+                                           uint32_t stn)
+    {
+        const uint32_t v0 = A & B, f0 = v0 & ~(v0 << 1), bnd = st | stn | 1u;
+        const uint32_t y = ~f0 & ~(bnd >> 1);
+        e0 = ((y + bnd) & ~y) & f0;
+        el = A & ((B << 1) | (Bl >> 31)) & ~B & ~(A << 1);
+        er = A & ((B >> 1) | (Br << 31)) & ~B & ~(A >> 1);
+    }
+    __device__ __forceinline__ uint32_t any() const { return e0 | el | er; }
+    // pop the next contact: its own column x, returns the neighbour column (-1..32)
+    __device__ __forceinline__ int pop(uint32_t &xo)
+    {
+        const uint32_t x = __builtin_ctz(any()), bit = 1u << x;
+        const bool tl = (el & bit) != 0, t0 = (e0 & bit) != 0;
+        el ^= tl ? bit : 0u;
+        e0 ^= (!tl && t0) ? bit : 0u;
+        er ^= (!tl && !t0) ? bit : 0u;
+        xo = x;
+        return (int)x + (tl ? -1 : (t0 ? 0 : 1));
+    }
+};
+
+// Nodes of a row of segments: the run containing column p of segment j (p in
+// -1..32 reaches into the segments to the left / right).
+struct SegNodes {
+    uint32_t st;      // run starts of segment j
+    uint32_t base;    // local index of segment j's first block
+    uint32_t first;   // node standing for segment j's first run
+    uint32_t left;    // node of the left segment's last run
+    uint32_t right;   // node standing for the right segment's first run
+    __device__ __forceinline__ uint32_t node(int p) const
+    {
+        if (p < 0) return left;
+        if (p > 31) return right;
+        const uint32_t s = hibit(st & upto((uint32_t)p));
+        return s ? base + (s >> 1) : first;
+    }
+};
+
+// Node of the last run of a segment with run mask o (o != 0) whose first block
+// is base, `first` standing for its first run.
+__device__ __forceinline__ uint32_t last_run_node(uint32_t o, uint32_t base, uint32_t first)
+{
+    const uint32_t s = o ? hibit(o & ~(o << 1)) : 0u;
+    return s ? base + (s >> 1) : first;
+}
+
+// Chase the nodes base + k for the bits k of pend to their roots and store
+// kResolved | glob(root) in their entries (glob: local -> global block index).
+// A chase stops at an entry another lane already resolved.  No unions may run
+// concurrently.
+template <class Glob>
+__device__ __forceinline__ void resolve_nodes(uint32_t *par, uint32_t base, uint32_t pend, const Glob &glob)
+{
+    uint32_t cur = 0, orig = 0;
+    bool busy = false;
+    for (;;) {
+        if (!busy) {
+            if (!pend) break;
+            orig = cur = base + __builtin_ctz(pend);
+            pend &= pend - 1;
+            busy = true;
+        }
+        const uint32_t p = lds_load(&par[cur]);
+        uint32_t g;
+        if (p & kResolved) {
+            g = p;
+        } else if (p == cur) {
+            g = kResolved | glob(cur);
+        } else {
+            cur = p;
+            continue;
+        }
+        lds_store(&par[orig], g);
+        busy = false;
+    }
+}
+
 // Nonzero-byte test: bit 8k of the result is set iff byte k of v is nonzero.
 __device__ __forceinline__ uint32_t nz_bytes(uint32_t v)
 {
