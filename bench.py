@@ -354,7 +354,7 @@ def main():
     ap.add_argument("--variant", default="", help="TILE,FLATTEN: run this bleq 2D kernel variant, no autotune")
     ap.add_argument("--quick", action="store_true", help="headline 2D line only")
     ap.add_argument("--skip", default="", help="comma list of extras to skip: 3d,batch,serve")
-    ap.add_argument("--serve-qps", type=float, default=200.0)
+    ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-budget", type=int, default=4096, help="MiB")

@@ -10,8 +10,8 @@
 // pass; here three launches (and a one-wave count) resolve everything with
 // union-find:
 //
-//   1. tile    one workgroup (512 threads) per 32x16x16-block tile
-//              (64x32x32 voxels): each lane turns four 32-voxel rows into bit
+//   1. tile    one workgroup (512 threads) per 64x16x8-block tile
+//              (128x32x16 voxels): each lane turns four 32-voxel rows into bit
 //              masks; union-find over the runs of each 2x2 bar of voxel rows
 //              in LDS; writes the tile-local root (global block index) and the
 //              mask of every block.
@@ -28,7 +28,7 @@
 namespace lbl {
 namespace {
 
-constexpr uint32_t TX = 32, TY = 16, TZ = 16;     // tile, blocks (64 x 32 x 32 voxels)
+constexpr uint32_t TX = 64, TY = 16, TZ = 8;      // tile, blocks (128 x 32 x 16 voxels)
 constexpr uint32_t FACE_Z = TX * TY, FACE_Y = TX * (TZ - 2), FACE_X = (TY - 2) * (TZ - 2);
 constexpr uint32_t JOBS = 2 * FACE_Z + 2 * FACE_Y + 2 * FACE_X;
 constexpr uint32_t FINAL_BLOCKS = 2;                // one 16-byte store per voxel row per thread

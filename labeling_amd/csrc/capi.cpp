@@ -390,7 +390,9 @@ extern "C" lbl_status lbl_label2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *px, 
     if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t))) != LBL_OK)
         return st;
     hipStream_t s = ctx->stream;
-    LBL_HIP(ctx, hipMemcpy2DAsync(ctx->d_in, dpitch, px, pitch_bytes, w, h, hipMemcpyHostToDevice, s));
+    // the label buffer (4 B/px) stages the rows until the kernels write it
+    LBL_HIP(ctx, upload2d(ctx->d_in, dpitch, px, pitch_bytes, w, h, reinterpret_cast<uint8_t *>(ctx->d_lab),
+                          n * sizeof(uint32_t), s));
     LBL_HIP(ctx, hipEventRecord(ctx->ev0, s));
     Image2D im{ctx->d_in, dpitch, h, w, ctx->d_lab, ctx->d_misc};
     if ((st = run2d(ctx, alg, c, im, s)) != LBL_OK) return st;

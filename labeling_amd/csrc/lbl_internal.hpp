@@ -78,6 +78,13 @@ hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream
 hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 
+// Copy h rows of w bytes from host memory (row pitch spitch) to device rows of
+// pitch dpitch on stream s.  Unequal pitches go through `staging` (device
+// memory, at least spitch * (h - 1) + w bytes; otherwise a 2D copy is used)
+// and a re-pitch kernel.
+hipError_t upload2d(uint8_t *dst, size_t dpitch, const uint8_t *src, size_t spitch, uint32_t w, uint32_t h,
+                    uint8_t *staging, size_t staging_bytes, hipStream_t s);
+
 // Limits checked on the host before any launch.
 constexpr uint64_t kMaxBlocks2D = (1ull << 28);   // 2D block word packs parent << 4 | mask
 constexpr uint64_t kMaxElems = (1ull << 32) - 2;  // labels are uint32 with 0 = background

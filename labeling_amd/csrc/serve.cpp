@@ -160,7 +160,7 @@ void lbl_server::launch(Group *g)
             uint32_t *d_lab = reinterpret_cast<uint32_t *>(d_in + g->reqs[i]->in_bytes());
             uint32_t *d_P = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(d_lab) + g->reqs[i]->lab_bytes());
             cur += g->reqs[i]->bytes();
-            (void)hipMemcpy2DAsync(d_in, pitch, q.px, q.w, q.w, q.h, hipMemcpyHostToDevice, s);
+            (void)upload2d(d_in, pitch, q.px, q.w, q.w, q.h, reinterpret_cast<uint8_t *>(d_lab), g->reqs[i]->lab_bytes(), s);
             Geom2D &gm = sl.h_geoms[i];
             gm = Geom2D{};
             gm.px = d_in;
@@ -198,7 +198,7 @@ void lbl_server::launch(Group *g)
             else (void)lbeq3d_run(im, opt, ws, s, nullptr);
         } else {
             const uint32_t pitch = (q.w + 15u) & ~15u;
-            (void)hipMemcpy2DAsync(d_in, pitch, q.px, q.w, q.w, q.h, hipMemcpyHostToDevice, s);
+            (void)upload2d(d_in, pitch, q.px, q.w, q.w, q.h, reinterpret_cast<uint8_t *>(d_lab), rq->lab_bytes(), s);
             Image2D im{d_in, pitch, q.h, q.w, d_lab, d_counts};
             (void)lbeq2d_run(im, q.conn == 8 ? 8 : 4, opt, ws, s, nullptr);
         }
