@@ -109,11 +109,11 @@ size_t plan(lbl_batch *b, uint32_t tile)
         bleq2d_geometry(g, tile);
         g.tile_base = tb;
         g.final_base = fb;
-        g.P = reinterpret_cast<uint32_t *>(off);   // offsets, rebased once ws exists
-        g.slots = reinterpret_cast<uint32_t *>(off + align256((uint64_t)g.bw * g.bh * sizeof(uint32_t)));
+        g.slots = reinterpret_cast<uint32_t *>(off);   // offsets, rebased once ws exists
+        g.P = reinterpret_cast<uint32_t *>(off + align256(kCountBytes));
         tb += g.ntiles;
         fb += g.nfinal;
-        off += align256((uint64_t)g.bw * g.bh * sizeof(uint32_t)) + kCountBytes;
+        off += align256(kCountBytes) + align256((uint64_t)g.bw * g.bh * sizeof(uint32_t));
         b->geoms[i] = g;
     }
     b->total_tiles = tb;
@@ -197,6 +197,7 @@ lbl_status alloc_ws(lbl_batch *b, size_t bytes)
         b->ws = nullptr;
     }
     B_HIP(b->ctx, hipMalloc(&b->ws, std::max<size_t>(bytes, 256)));
+    B_HIP(b->ctx, hipMemset(b->ws, 0, std::max<size_t>(bytes, 256)));   // count slots start at zero
     return LBL_OK;
 }
 
@@ -342,6 +343,7 @@ extern "C" lbl_status lbl_autotune2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d
         }
         B_HIP(ctx, hipMalloc(&ctx->ws, need));
         ctx->ws_bytes = need;
+        B_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, ctx->stream));   // count slots start at zero
     }
     Image2D im{d_px, pitch_bytes, h, w, d_labels, ctx->d_misc};
     Options opt = ctx->opt;

@@ -7,8 +7,8 @@
 // foreground pixel of one is 8-adjacent to a foreground pixel of the other, and
 // each component is labelled by its minimum block.  The reference iterates
 // Scan/Analyze passes over the whole block grid with a blocking host round trip
-// per pass; here the equivalences are resolved by union-find in four launches
-// and no host synchronisation:
+// per pass; here the equivalences are resolved by union-find in three or four
+// launches and no host synchronisation:
 //
 //   1. tile    one workgroup per TBW x TBH-block tile: each lane turns two
 //              32-pixel row segments into bit masks; union-find over the
@@ -22,8 +22,11 @@
 //              block: the block and its tile-local root are pointed straight
 //              at the global root.
 //   4. final   one thread per 2 blocks: root of the block, writes the labels
-//              of the 2x2 pixels of each block, 0 for background, and counts
-//              roots into 64 spread slots, summed by a one-wave count kernel.
+//              of the 2x2 pixels of each block, 0 for background.
+//
+// Components are counted at the tile-local roots by (1), each link made in (2)
+// subtracts one, and the image's first workgroup of (4) sums the 64 spread
+// count slots (zeroing them for the next run).
 //
 // Every kernel reads its image geometry from a "source": One (a single image
 // passed by value) or Many (a device table of images; a workgroup finds its
@@ -67,12 +70,6 @@ struct One {
         lw = gw;
         return g;
     }
-    __device__ __forceinline__ const Geom2D &by_image(uint32_t i, uint32_t &li) const
-    {
-        li = i;
-        return g;
-    }
-    uint32_t images() const { return g.count ? 1u : 0u; }
 };
 
 struct Many {
@@ -103,12 +100,6 @@ struct Many {
         lw = gw - r.final_base;
         return r;
     }
-    __device__ __forceinline__ const Geom2D &by_image(uint32_t i, uint32_t &li) const
-    {
-        li = i;
-        return g[i];
-    }
-    uint32_t images() const { return n; }
 };
 
 // ---- 1. tile kernel -------------------------------------------------------------
@@ -152,7 +143,6 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
     const uint32_t t = threadIdx.x, r = t / LPB, j = t % LPB;
     const uint32_t bx0 = (lt % G.tiles_x) * TBW, by0 = (lt / G.tiles_x) * TBH;
     const uint32_t by = by0 + r, bxs = bx0 + j * LANE_BLOCKS;
-    if (lt == 0 && G.count) count_clear(G.slots);
 
     uint32_t a = 0, b = 0;
     if (by < bh && bxs < bw) {
@@ -237,12 +227,17 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
         load_nodes();
     }
 
-    // block words: root << 4 | mask (background: own index << 4)
+    // block words: root << 4 | mask (background: own index << 4); the tile's
+    // components are counted at their roots (the merge subtracts its links)
+    uint32_t roots = 0;
     if (by < bh && bxs < bw) {
         uint32_t out[LANE_BLOCKS], cur = 0;
 #pragma unroll
         for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {
-            if (st & (3u << (2 * kk))) cur = (v[kk] & kResolved) ? v[kk] : g0 + kk;
+            if (st & (3u << (2 * kk))) {
+                cur = (v[kk] & kResolved) ? v[kk] : g0 + kk;
+                roots += (cur & ~kResolved) == g0 + kk;
+            }
             const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
             out[kk] = m ? (cur << 4) | m : (g0 + kk) << 4;   // << 4 drops the tag bit
         }
@@ -255,6 +250,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
             for (uint32_t kk = 0; kk < LANE_BLOCKS && bxs + kk < bw; ++kk) dst[kk] = out[kk];
         }
     }
+    if (G.count) count_add(G.slots, roots);   // one image per workgroup: uniform branch
 }
 
 // ---- 2./3. tile-border kernels -----------------------------------------------------
@@ -331,7 +327,9 @@ __global__ __launch_bounds__(256) void bleq2d_merge(Src src, uint32_t total_tile
     const uint64_t key = np ? ((uint64_t)(vb >> 4) << 32) | pr[0] : 0;
     uint32_t leader;
     const bool lead = run_leader(key, valid ? G->tile_base : 0u, np != 0, leader);
-    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) Forest::unite(G->P, vb >> 4, pr[i]);   // tile-local roots
+    uint32_t links = 0;
+    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) links += Forest::unite(G->P, vb >> 4, pr[i]);   // tile-local roots
+    count_links(valid && G->count ? G->slots : nullptr, valid && G->count ? links : 0u);
 }
 
 // Point each distinct tile-local root seen on the tile border (and the border
@@ -421,11 +419,9 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
     uint32_t key_root = lead ? Forest::find_final(P, key) : 0u;
     key_root = __shfl(key_root, (int)leader, 64);
 
-    uint32_t roots = 0;
     if (inside) {
         uint32_t lab[FINAL_BLOCKS];
         uint32_t prev_parent = key, prev_root = key_root;
-        const uint32_t b0 = by * bw + bxs;
         const bool ref = (G.flags & kRefLabels) != 0;
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
@@ -435,7 +431,6 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
             const uint32_t root = p == prev_parent ? prev_root : Forest::find_final(P, p);
             prev_parent = p;
             prev_root = root;
-            roots += root == b0 + k;
             // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
             lab[k] = ref ? 1u + (root % bw) + (root / bw) * G.ref_bw : root + 1u;
         }
@@ -455,15 +450,8 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
         store_row4(row0, top, x0, w);
         if (two_rows) store_row4(row1, bot, x0, w);
     }
-    if (G.count) count_add(G.slots, roots);   // one image per workgroup: uniform branch
-}
-
-template <class Src>
-__global__ __launch_bounds__(kCountSlots) void bleq2d_count(Src src)
-{
-    uint32_t li;
-    const Geom2D &G = src.by_image(blockIdx.x, li);
-    if (G.count) count_reduce(G.slots, G.count);
+    // the tile and merge kernels have finished: the image's first workgroup sums its count
+    if (lw == 0 && G.count && threadIdx.x < kCountSlots) count_reduce(G.slots, G.count);
 }
 
 // ---- host side --------------------------------------------------------------------
@@ -487,10 +475,6 @@ struct Variant {
         }
         hipLaunchKernelGGL((bleq2d_final<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
         if (kt) kt->mark(s, "bleq2d_final");
-        if (src.images()) {
-            hipLaunchKernelGGL((bleq2d_count<Src>), dim3(src.images()), dim3(kCountSlots), 0, s, src);
-            if (kt) kt->mark(s, "bleq2d_count");
-        }
         return hipGetLastError();
     }
 };
@@ -533,7 +517,7 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
 size_t bleq2d_workspace(uint32_t h, uint32_t w)
 {
     const uint64_t bw = (w + 1ull) / 2, bh = (h + 1ull) / 2;
-    return align256(bw * bh * sizeof(uint32_t)) + kCountBytes;
+    return align256(kCountBytes) + align256(bw * bh * sizeof(uint32_t));
 }
 
 hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
@@ -547,10 +531,10 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
     g.w = im.w;
     g.labels = im.labels;
     g.count = im.count;
-    g.P = static_cast<uint32_t *>(ws);
+    g.slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
+    g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
     g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
     bleq2d_geometry(g, opt.tile2d);
-    g.slots = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256((size_t)g.bw * g.bh * 4));
     if (kt) kt->begin(s);
     return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten);
 }

@@ -18,7 +18,7 @@ struct Geom2D {
     uint32_t *labels;
     uint32_t *count;       // component count (may be null)
     uint32_t *P;           // block words: bw * bh uint32
-    uint32_t *slots;       // kCountBytes of partial counts (workspace, after P)
+    uint32_t *slots;       // kCountBytes of partial counts (workspace, before P; zero between runs)
     uint64_t pitch;
     uint32_t h, w, bw, bh;
     uint32_t tiles_x, ntiles;

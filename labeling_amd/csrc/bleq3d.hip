@@ -7,8 +7,9 @@
 // equivalent when a foreground voxel of one is 26-adjacent to a foreground voxel
 // of the other, and each component is labelled by its minimum block.  The
 // reference iterates whole-grid Scan/Analyze passes with a host round trip per
-// pass; here three launches (and a one-wave count) resolve everything with
-// union-find:
+// pass; here three launches resolve everything with union-find (components
+// counted at tile-local roots, minus the merge's links, summed by the final
+// kernel's first workgroup):
 //
 //   1. tile    one workgroup (512 threads) per 64x16x8-block tile
 //              (128x32x16 voxels): each lane turns four 32-voxel rows into bit
@@ -118,7 +119,6 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     const uint32_t tx = blockIdx.x % tiles_x, ty = (blockIdx.x / tiles_x) % tiles_y, tz = blockIdx.x / (tiles_x * tiles_y);
     const uint32_t bx0 = tx * TBW, by0 = ty * TBY, bz0 = tz * TBZ;
     const uint32_t bxs = bx0 + j * kLaneBlocks, by = by0 + ly, bz = bz0 + lz;
-    if (blockIdx.x == 0 && count) count_clear(slots);
 
     uint32_t a = 0, b = 0, c = 0, e = 0;
     if (bz < bd && by < bh && bxs < bw) {
@@ -231,12 +231,17 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         load_nodes();
     }
 
-    // block words (root, own index for background) and masks (bit dx + 2 dy + 4 dz)
+    // block words (root, own index for background) and masks (bit dx + 2 dy + 4 dz);
+    // the tile's components are counted at their roots (the merge subtracts its links)
+    uint32_t roots = 0;
     if (bz < bd && by < bh && bxs < bw) {
         uint32_t out[kLaneBlocks], mk[kLaneBlocks / 4] = {0, 0, 0, 0}, cur = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kLaneBlocks; ++k) {
-            if (st & (3u << (2 * k))) cur = (v[k] & kResolved) ? (v[k] & ~kResolved) : g0 + k;
+            if (st & (3u << (2 * k))) {
+                cur = (v[k] & kResolved) ? (v[k] & ~kResolved) : g0 + k;
+                roots += cur == g0 + k;
+            }
             const uint32_t m = ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
                                (((e >> (2 * k)) & 3u) << 6);
             out[k] = m ? cur : g0 + k;
@@ -257,6 +262,7 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
             }
         }
     }
+    if (count) count_add(slots, roots);
 }
 
 // ---- 2. tile-face merge -----------------------------------------------------
@@ -309,7 +315,8 @@ __device__ __forceinline__ bool job_block(const Grid3 &g, uint32_t gid, uint32_t
 // other tiles, between tile-local roots.  Distinct root pairs only; the first
 // pair is skipped by every lane whose left neighbour lane holds the same one
 // (consecutive face blocks along x usually repeat it).
-__global__ __launch_bounds__(256) void bleq3d_merge(uint32_t *P, const uint8_t *__restrict__ M, Grid3 g)
+__global__ __launch_bounds__(256) void bleq3d_merge(uint32_t *P, const uint8_t *__restrict__ M, Grid3 g,
+                                                   uint32_t *count, uint32_t *slots)
 {
     uint32_t bx = 0, by = 0, bz = 0, bx0 = 0, by0 = 0, bz0 = 0;
     const bool inside = job_block(g, blockIdx.x * blockDim.x + threadIdx.x, bx, by, bz, bx0, by0, bz0);
@@ -337,7 +344,9 @@ __global__ __launch_bounds__(256) void bleq3d_merge(uint32_t *P, const uint8_t *
     const uint64_t key = np ? ((uint64_t)rb << 32) | pr[0] : 0;
     uint32_t leader;
     const bool lead = run_leader(key, 0u, np != 0, leader);
-    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) Forest::unite(P, rb, pr[i]);
+    uint32_t links = 0;
+    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) links += Forest::unite(P, rb, pr[i]);
+    if (count) count_links(slots, links);   // kernel-uniform branch
 }
 
 // ---- 3. final kernel --------------------------------------------------------------
@@ -383,7 +392,6 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
     uint32_t key_root = lead ? Forest::find_final(P, key) : 0u;
     key_root = __shfl(key_root, (int)leader, 64);
 
-    uint32_t roots = 0;
     if (inside) {
         uint32_t lab[FINAL_BLOCKS];
         uint32_t prev_parent = key, prev_root = key_root;
@@ -394,7 +402,6 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
             const uint32_t root = pv[k] == prev_parent ? prev_root : Forest::find_final(P, pv[k]);
             prev_parent = pv[k];
             prev_root = root;
-            roots += root == (uint32_t)(b0 + k);
             lab[k] = root + 1u;
         }
         const uint32_t x0 = 2 * bxs;
@@ -418,7 +425,8 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
             }
         }
     }
-    if (count) count_add(slots, roots);
+    // the tile and merge kernels have finished: the first workgroup sums the count
+    if (blockIdx.x == 0 && count && threadIdx.x < kCountSlots) count_reduce(slots, count);
 }
 
 }  // namespace
@@ -426,7 +434,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
 size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w)
 {
     const uint64_t nb = ((d + 1ull) / 2) * ((h + 1ull) / 2) * ((w + 1ull) / 2);
-    return align256(nb * sizeof(uint32_t)) + align256(nb) + kCountBytes;
+    return align256(kCountBytes) + align256(nb * sizeof(uint32_t)) + align256(nb);
 }
 
 hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t s, KernelTimer *kt)
@@ -436,9 +444,9 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     Grid3 g{bd, bh, bw, (bw + TX - 1) / TX, (bh + TY - 1) / TY, 0};
     const uint32_t tiles_z = (bd + TZ - 1) / TZ;
     g.ntiles = g.tiles_x * g.tiles_y * tiles_z;
-    uint32_t *P = static_cast<uint32_t *>(ws);
-    uint8_t *M = static_cast<uint8_t *>(ws) + align256(nb * sizeof(uint32_t));
-    uint32_t *slots = reinterpret_cast<uint32_t *>(M + align256(nb));
+    uint32_t *slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
+    uint32_t *P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
+    uint8_t *M = reinterpret_cast<uint8_t *>(P) + align256(nb * sizeof(uint32_t));
     const bool vec_in = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
     const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
     const uint32_t face_grid = (uint32_t)(((uint64_t)g.ntiles * JOBS + 255) / 256);
@@ -453,7 +461,7 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
                            im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_tile");
-    hipLaunchKernelGGL(bleq3d_merge, dim3(face_grid), dim3(256), 0, s, P, M, g);
+    hipLaunchKernelGGL(bleq3d_merge, dim3(face_grid), dim3(256), 0, s, P, M, g, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_merge");
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
     if (vec_out)
@@ -463,10 +471,6 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
         hipLaunchKernelGGL(bleq3d_final<false>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd,
                            bh, bw, xgroups, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_final");
-    if (im.count) {
-        hipLaunchKernelGGL(count_reduce_kernel<0>, dim3(1), dim3(kCountSlots), 0, s, slots, im.count);
-        if (kt) kt->mark(s, "count_reduce");
-    }
     return hipGetLastError();
 }
 

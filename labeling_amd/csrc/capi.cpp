@@ -53,8 +53,9 @@ struct DeviceGuard {
 };
 
 // Grow a cached device buffer; waits for the device before freeing the old one.
-lbl_status ensure(lbl_ctx *ctx, void **buf, size_t *cap, size_t need)
+lbl_status ensure(lbl_ctx *ctx, void **buf, size_t *cap, size_t need, bool *fresh)
 {
+    if (fresh) *fresh = false;
     if (*cap >= need && *buf) return LBL_OK;
     if (*buf) {
         LBL_HIP(ctx, hipDeviceSynchronize());
@@ -71,6 +72,7 @@ lbl_status ensure(lbl_ctx *ctx, void **buf, size_t *cap, size_t need)
                                          hipGetErrorString(e));
     }
     *cap = need;
+    if (fresh) *fresh = true;
     return LBL_OK;
 }
 
@@ -292,8 +294,11 @@ extern "C" int lbl_last_kernel_times(lbl_ctx *ctx, float *us, const char **names
 
 static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s)
 {
-    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w));
+    bool fresh = false;
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w), &fresh);
     if (st != LBL_OK) return st;
+    // every workspace starts with the count slots, which runs leave at zero
+    if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
     KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
     Options opt = ctx->opt;
     auto it = ctx->tuning.find(TuneKey{(int)alg, 0, size_class_of((uint64_t)im.h * im.w)});
@@ -308,8 +313,11 @@ static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, 
 
 static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_t s)
 {
-    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace3d(alg, im.d, im.h, im.w));
+    bool fresh = false;
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace3d(alg, im.d, im.h, im.w), &fresh);
     if (st != LBL_OK) return st;
+    // every workspace starts with the count slots, which runs leave at zero
+    if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
     KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
     hipError_t e = alg == LBL_BLEQ ? bleq3d_run(im, ctx->opt, ctx->ws, s, kt) : lbeq3d_run(im, ctx->opt, ctx->ws, s, kt);
     if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq3d launch" : "lbeq3d launch", e);
@@ -386,8 +394,8 @@ extern "C" lbl_status lbl_label2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *px, 
     DeviceGuard g(ctx->device);
     const size_t n = (size_t)h * w;
     const size_t dpitch = (w + 15u) & ~size_t(15);   // 16-byte rows for vector loads
-    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_in), &ctx->in_bytes, dpitch * h)) != LBL_OK) return st;
-    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t))) != LBL_OK)
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_in), &ctx->in_bytes, dpitch * h, nullptr)) != LBL_OK) return st;
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t), nullptr)) != LBL_OK)
         return st;
     hipStream_t s = ctx->stream;
     // the label buffer (4 B/px) stages the rows until the kernels write it
@@ -411,8 +419,8 @@ extern "C" lbl_status lbl_label3d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *vox,
     if ((st = check_mode(ctx, alg, true)) != LBL_OK) return st;
     DeviceGuard g(ctx->device);
     const size_t n = (size_t)d * h * w;
-    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_in), &ctx->in_bytes, n)) != LBL_OK) return st;
-    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t))) != LBL_OK)
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_in), &ctx->in_bytes, n, nullptr)) != LBL_OK) return st;
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t), nullptr)) != LBL_OK)
         return st;
     hipStream_t s = ctx->stream;
     LBL_HIP(ctx, hipMemcpyAsync(ctx->d_in, vox, n, hipMemcpyHostToDevice, s));
@@ -440,8 +448,10 @@ extern "C" lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
         need = std::max(need, workspace2d(alg, im.h, im.w));
     }
     DeviceGuard g(ctx->device);
-    if ((st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, need)) != LBL_OK) return st;
     hipStream_t s = pick_stream(ctx, stream);
+    bool fresh = false;
+    if ((st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, need, &fresh)) != LBL_OK) return st;
+    if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
     for (uint32_t i = 0; i < n; ++i) {
         const lbl_image2d &im = images[i];
         Image2D one{im.px, im.pitch_bytes, im.h, im.w, im.labels, d_ncomp ? d_ncomp + i : nullptr};

@@ -85,6 +85,12 @@ hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream
 hipError_t upload2d(uint8_t *dst, size_t dpitch, const uint8_t *src, size_t spitch, uint32_t w, uint32_t h,
                     uint8_t *staging, size_t staging_bytes, hipStream_t s);
 
+// Component counts: kCountSlots partial counters kCountStride words apart (a
+// 64-byte line each) at the start of every workspace; zero between runs.
+constexpr uint32_t kCountSlots = 64;
+constexpr uint32_t kCountStride = 16;   // words between slots
+constexpr size_t kCountBytes = kCountSlots * kCountStride * sizeof(uint32_t);
+
 // Limits checked on the host before any launch.
 constexpr uint64_t kMaxBlocks2D = (1ull << 28);   // 2D block word packs parent << 4 | mask
 constexpr uint64_t kMaxElems = (1ull << 32) - 2;  // labels are uint32 with 0 = background

@@ -158,7 +158,8 @@ void lbl_server::launch(Group *g)
             const uint32_t pitch = (q.w + 15u) & ~15u;
             uint8_t *d_in = base + cur;
             uint32_t *d_lab = reinterpret_cast<uint32_t *>(d_in + g->reqs[i]->in_bytes());
-            uint32_t *d_P = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(d_lab) + g->reqs[i]->lab_bytes());
+            uint8_t *ws = reinterpret_cast<uint8_t *>(d_lab) + g->reqs[i]->lab_bytes();
+            uint32_t *d_P = reinterpret_cast<uint32_t *>(ws + align256(kCountBytes));
             cur += g->reqs[i]->bytes();
             (void)upload2d(d_in, pitch, q.px, q.w, q.w, q.h, reinterpret_cast<uint8_t *>(d_lab), g->reqs[i]->lab_bytes(), s);
             Geom2D &gm = sl.h_geoms[i];
@@ -172,7 +173,8 @@ void lbl_server::launch(Group *g)
             gm.w = q.w;
             gm.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
             bleq2d_geometry(gm, 0);
-            gm.slots = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(d_P) + align256((size_t)gm.bw * gm.bh * 4));
+            gm.slots = reinterpret_cast<uint32_t *>(ws);   // count slots first, zeroed below
+            (void)hipMemsetAsync(ws, 0, kCountBytes, s);
             gm.tile_base = tb;
             gm.final_base = fb;
             tb += gm.ntiles;
@@ -191,6 +193,7 @@ void lbl_server::launch(Group *g)
         uint8_t *d_in = base + cur;
         uint32_t *d_lab = reinterpret_cast<uint32_t *>(d_in + rq->in_bytes());
         void *ws = reinterpret_cast<uint8_t *>(d_lab) + rq->lab_bytes();
+        (void)hipMemsetAsync(ws, 0, kCountBytes, s);   // count slots (workspace offset 0) start at zero
         if (rq->is3d()) {
             (void)hipMemcpyAsync(d_in, q.px, rq->elems(), hipMemcpyHostToDevice, s);
             Image3D im{d_in, q.d, q.h, q.w, d_lab, d_counts};

@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "lbl_internal.hpp"
+
 #include <cstdint>
 
 namespace lbl {
@@ -140,17 +142,19 @@ struct PackedForest {
         return x;
     }
 
-    __device__ static void unite(uint32_t *w, uint32_t a, uint32_t b)
+    // Unite the sets of a and b; true iff this call made the link that joined
+    // two different sets (exactly one call per merge of two sets returns true).
+    __device__ static bool unite(uint32_t *w, uint32_t a, uint32_t b)
     {
         for (;;) {
             a = find(w, a);
             b = find(w, b);
-            if (a == b) return;
+            if (a == b) return false;
             if (a < b) { uint32_t t = a; a = b; b = t; }
             uint32_t va = dev_load(&w[a]);
             if ((va >> SHIFT) != a) continue;   // a was linked meanwhile: re-find
             uint32_t old = dev_min(&w[a], (b << SHIFT) | (va & kLowMask));
-            if ((old >> SHIFT) == a) return;
+            if ((old >> SHIFT) == a) return true;
             a = old >> SHIFT;
         }
     }
@@ -227,9 +231,6 @@ __device__ __forceinline__ void store_nt2(uint32_t *p, uint32_t a, uint32_t b)
 // thousands of atomics on one L2 line, so counts go to kCountSlots words 64 B
 // apart (a workgroup adds its total to slot blockIdx % kCountSlots) and
 // count_reduce sums the slots afterwards.
-constexpr uint32_t kCountSlots = 64;
-constexpr uint32_t kCountStride = 16;   // words between slots
-constexpr size_t kCountBytes = kCountSlots * kCountStride * sizeof(uint32_t);
 
 // Workgroup-wide sum of v, added to slots[blockIdx % kCountSlots] by thread 0.
 // Every thread of the workgroup must call it.
@@ -247,16 +248,42 @@ __device__ __forceinline__ void count_add(uint32_t *slots, uint32_t v)
     }
 }
 
-// Sum the slots into *count (called by one wave of kCountSlots threads).
-__device__ __forceinline__ void count_reduce(const uint32_t *slots, uint32_t *count)
+// Sum the slots into *count and zero them again (called by one full wave,
+// threads 0..63, after every kernel that adds to them has finished).  Slots
+// are zero between pipeline runs: workspaces zero them when allocated and
+// every run that counts ends here.
+__device__ __forceinline__ void count_reduce(uint32_t *slots, uint32_t *count)
 {
-    uint32_t v = threadIdx.x < kCountSlots ? slots[threadIdx.x * kCountStride] : 0u;
+    uint32_t v = slots[threadIdx.x * kCountStride];
+    slots[threadIdx.x * kCountStride] = 0;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (threadIdx.x == 0) *count = v;
 }
 
+// n links each joined two components: the count of the image whose slots
+// these are drops by n (unsigned slot arithmetic wraps; only the sum over the
+// slots is meaningful).  The lanes of a wave add up first when they all count
+// for one image (one atomic per wave); slots may be null where n is 0.  All
+// lanes of the wave must call it.
+__device__ __forceinline__ void count_links(uint32_t *slots, uint32_t n)
+{
+    const uint64_t has = __ballot(n != 0);
+    if (!has) return;
+    const uint32_t lane = threadIdx.x & 63u, first = (uint32_t)__builtin_ctzll(has);
+    const uint64_t key = reinterpret_cast<uintptr_t>(slots);
+    const uint64_t key0 = __shfl(key, (int)first, 64);
+    uint32_t *slot = &slots[(blockIdx.x % kCountSlots) * kCountStride];
+    if (__ballot(n != 0 && key != key0) == 0) {
+        uint32_t sum = n;
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        if (lane == first) atomicSub(slot, sum);
+    } else if (n) {
+        atomicSub(slot, n);
+    }
+}
+
 template <int Dummy = 0>
-__global__ __launch_bounds__(kCountSlots) void count_reduce_kernel(const uint32_t *slots, uint32_t *count)
+__global__ __launch_bounds__(kCountSlots) void count_reduce_kernel(uint32_t *slots, uint32_t *count)
 {
     count_reduce(slots, count);
 }
