@@ -16,8 +16,10 @@
 //              masks; union-find over the runs of each 2x2 bar of voxel rows
 //              in LDS; writes the tile-local root (global block index) and the
 //              mask of every block.
-//   2. merge   one thread per block on a tile face: unions with each earlier
-//              26-neighbour block in another tile that it touches.
+//   2. merge   unions across tile faces: bar segments against their earlier
+//              neighbour bars in other tiles (bit-mask contacts), and the
+//              tile's first block column against the column to its left;
+//              pairs of tile-local roots queued in LDS, united by all threads.
 //   3. final   one thread per 2 blocks: root of each block (neighbouring lanes
 //              share one chase per tile-local root), writes the 16 voxel
 //              labels of the 2 blocks, counts roots.
@@ -30,8 +32,12 @@ namespace lbl {
 namespace {
 
 constexpr uint32_t TX = 64, TY = 16, TZ = 8;      // tile, blocks (128 x 32 x 16 voxels)
-constexpr uint32_t FACE_Z = TX * TY, FACE_Y = TX * (TZ - 2), FACE_X = (TY - 2) * (TZ - 2);
-constexpr uint32_t JOBS = 2 * FACE_Z + 2 * FACE_Y + 2 * FACE_X;
+// merge lanes per tile: segments of the bars whose earlier neighbour bars lie
+// in another tile (z = 0 plane, and the y = 0 / y = TY - 1 rows above it), and
+// one lane per bar for the tile's first block column
+constexpr uint32_t MERGE_BARS = TY + 2 * (TZ - 1);
+constexpr uint32_t BAR_LANES = MERGE_BARS * (TX / kLaneBlocks);
+constexpr uint32_t X_LANES = TY * TZ;
 constexpr uint32_t FINAL_BLOCKS = 2;                // one 16-byte store per voxel row per thread
 constexpr uint32_t NT_FINAL = 256;
 
@@ -265,87 +271,151 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     if (count) count_add(slots, roots);
 }
 
-// ---- 2. tile-face merge -----------------------------------------------------
-
-__device__ __forceinline__ void face_block(uint32_t j, uint32_t &lx, uint32_t &ly, uint32_t &lz)
-{
-    if (j < 2 * FACE_Z) {
-        lz = j < FACE_Z ? 0 : TZ - 1;
-        const uint32_t r = j % FACE_Z;
-        ly = r / TX;
-        lx = r % TX;
-        return;
-    }
-    j -= 2 * FACE_Z;
-    if (j < 2 * FACE_Y) {
-        ly = j < FACE_Y ? 0 : TY - 1;
-        const uint32_t r = j % FACE_Y;
-        lz = 1 + r / TX;
-        lx = r % TX;
-        return;
-    }
-    j -= 2 * FACE_Y;
-    lx = j < FACE_X ? 0 : TX - 1;
-    const uint32_t r = j % FACE_X;
-    lz = 1 + r / (TY - 2);
-    ly = 1 + r % (TY - 2);
-}
+// ---- 2. tile-face merge -----------------------------------------------------------
+//
+// Unions across tile faces, between tile-local roots (P holds each block's
+// tile-local root, M its mask).  Two kinds of lanes:
+//   * bar lanes: a 16-block segment of a bar whose earlier neighbour bars lie
+//     in another tile (y-1, z-1, (y-1, z-1), (y+1, z-1)); the bars' voxel rows
+//     come back from M as bit masks and the tile kernel's contact logic finds
+//     the contacts (x-boundary contacts excepted: the next kind has them);
+//   * x lanes: one per bar at the tile's first block column, against the nine
+//     blocks at the column to the left.
+// Lanes push distinct pairs of roots into a workgroup queue in LDS, then every
+// thread of the workgroup unites pairs from the queue (a lane with many
+// contacts does not serialize its unions).
 
 struct Grid3 {
     uint32_t bd, bh, bw, tiles_x, tiles_y, ntiles;
 };
 
-__device__ __forceinline__ bool job_block(const Grid3 &g, uint32_t gid, uint32_t &bx, uint32_t &by, uint32_t &bz,
-                                          uint32_t &bx0, uint32_t &by0, uint32_t &bz0)
+constexpr uint32_t MERGE_THREADS = 256;
+constexpr uint32_t QUEUE_CAP = 8 * MERGE_THREADS;   // pairs per workgroup
+
+// The four voxel rows of 4 blocks (mask bit dx + 2 dy + 4 dz) as 8 bits each:
+// bit 2k + dx of row (dy, dz) for block k.
+__device__ __forceinline__ uint32_t row_bits8(uint32_t four_masks, uint32_t row_shift)
 {
-    const uint32_t tile = gid / JOBS, j = gid % JOBS;
-    if (tile >= g.ntiles) return false;
-    bx0 = (tile % g.tiles_x) * TX;
-    by0 = ((tile / g.tiles_x) % g.tiles_y) * TY;
-    bz0 = (tile / (g.tiles_x * g.tiles_y)) * TZ;
-    uint32_t lx, ly, lz;
-    face_block(j, lx, ly, lz);
-    bx = bx0 + lx;
-    by = by0 + ly;
-    bz = bz0 + lz;
-    return bx < g.bw && by < g.bh && bz < g.bd;
+    const uint32_t x = (four_masks >> row_shift) & 0x03030303u;
+    return (x | (x >> 6) | (x >> 12) | (x >> 18)) & 0xFFu;
 }
 
-// One thread per tile-face block: unions with its earlier 26-neighbours in
-// other tiles, between tile-local roots.  Distinct root pairs only; the first
-// pair is skipped by every lane whose left neighbour lane holds the same one
-// (consecutive face blocks along x usually repeat it).
-__global__ __launch_bounds__(256) void bleq3d_merge(uint32_t *P, const uint8_t *__restrict__ M, Grid3 g,
-                                                   uint32_t *count, uint32_t *slots)
+// Row masks a (y0, z0), b (y1, z0), c (y0, z1), e (y1, z1) of the 16-block
+// segment at m[0..15] (zero past bw).
+__device__ __forceinline__ void bar_rows(const uint8_t *m, uint32_t bxs, uint32_t bw, uint32_t &a, uint32_t &b,
+                                         uint32_t &c, uint32_t &e)
 {
-    uint32_t bx = 0, by = 0, bz = 0, bx0 = 0, by0 = 0, bz0 = 0;
-    const bool inside = job_block(g, blockIdx.x * blockDim.x + threadIdx.x, bx, by, bz, bx0, by0, bz0);
-    const uint32_t b = (bz * g.bh + by) * g.bw + bx;
-    const uint32_t m = inside ? M[b] : 0u;
-    const uint32_t rb = m ? dev_load(&P[b]) : 0u;   // tile-local root
-    // the 13 earlier neighbours: dz = -1 (9), dz = 0 & dy = -1 (3), dz = dy = 0 & dx = -1
-    uint32_t pr[13], np = 0;
-    if (m) {
-        for (int e = 0; e < 13; ++e) {
-            const int dz = e < 9 ? -1 : 0;
-            const int dy = e < 9 ? e / 3 - 1 : (e < 12 ? -1 : 0);
-            const int dx = e < 9 ? e % 3 - 1 : (e < 12 ? e - 10 : -1);
-            const uint32_t nx = bx + dx, ny = by + dy, nz = bz + dz;
-            if (nx >= g.bw || ny >= g.bh || nz >= g.bd) continue;
-            if (nx - bx0 < TX && ny - by0 < TY && nz - bz0 < TZ) continue;
-            const uint32_t n = (nz * g.bh + ny) * g.bw + nx;
-            if (!adjacent(m, M[n], dx, dy, dz)) continue;
-            const uint32_t rn = dev_load(&P[n]);
-            bool seen = rn == rb;
-            for (uint32_t i = 0; i < np; ++i) seen = seen || pr[i] == rn;
-            if (!seen) pr[np++] = rn;
+    uint32_t w[4];
+    if (bxs + kLaneBlocks <= bw && (reinterpret_cast<uintptr_t>(m) & 15u) == 0) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(m);
+        w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            w[i] = 0;
+            for (uint32_t k = 0; k < 4; ++k)
+                if (bxs + 4 * i + k < bw) w[i] |= (uint32_t)m[4 * i + k] << (8 * k);
         }
     }
-    const uint64_t key = np ? ((uint64_t)rb << 32) | pr[0] : 0;
-    uint32_t leader;
-    const bool lead = run_leader(key, 0u, np != 0, leader);
+    a = b = c = e = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        a |= row_bits8(w[i], 0) << (8 * i);
+        b |= row_bits8(w[i], 2) << (8 * i);
+        c |= row_bits8(w[i], 4) << (8 * i);
+        e |= row_bits8(w[i], 6) << (8 * i);
+    }
+}
+
+__global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const uint8_t *__restrict__ M, Grid3 g,
+                                                              uint32_t *count, uint32_t *slots)
+{
+    constexpr uint32_t LPB = TX / kLaneBlocks;
+    __shared__ uint2 queue[QUEUE_CAP];
+    __shared__ uint32_t queued;
+    if (threadIdx.x == 0) queued = 0;
+    __syncthreads();
+
+    const uint64_t gid = (uint64_t)blockIdx.x * MERGE_THREADS + threadIdx.x;
+    const uint64_t bar_jobs = (uint64_t)g.ntiles * BAR_LANES;
     uint32_t links = 0;
-    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) links += Forest::unite(P, rb, pr[i]);
+    uint32_t la = ~0u, lb = ~0u;   // the lane's previous pair
+    const auto push = [&](uint32_t ra, uint32_t rb) {
+        if (ra == rb || (ra == la && rb == lb)) return;
+        la = ra;
+        lb = rb;
+        const uint32_t i = atomicAdd(&queued, 1u);
+        if (i < QUEUE_CAP) queue[i] = make_uint2(ra, rb);
+        else links += Forest::unite(P, ra, rb);   // queue full: unite here
+    };
+
+    if (gid < bar_jobs) {
+        const uint32_t tile = (uint32_t)(gid / BAR_LANES), k = (uint32_t)(gid % BAR_LANES);
+        const uint32_t bar = k / LPB, j = k % LPB;
+        const uint32_t ly = bar < TY ? bar : ((bar - TY) & 1u) * (TY - 1), lz = bar < TY ? 0u : 1u + (bar - TY) / 2;
+        const uint32_t bx0 = (tile % g.tiles_x) * TX, by = ((tile / g.tiles_x) % g.tiles_y) * TY + ly;
+        const uint32_t bz = (tile / (g.tiles_x * g.tiles_y)) * TZ + lz, bxs = bx0 + j * kLaneBlocks;
+        if (by < g.bh && bz < g.bd && bxs < g.bw) {
+            const size_t row = ((size_t)bz * g.bh + by) * g.bw;
+            uint32_t a, b, c, e;
+            bar_rows(M + row + bxs, bxs, g.bw, a, b, c, e);
+            const uint32_t o = a | b | c | e, st = o & ~(o << 1);
+            for (uint32_t q = 0; q < 4 && o; ++q) {
+                // direction q: (y-1), (z-1), (y-1, z-1), (y+1, z-1); only those leaving the tile
+                const int dy = q == 1 ? 0 : (q == 3 ? 1 : -1), dz = q == 0 ? 0 : -1;
+                const bool out = (dy < 0 && ly == 0) || (dy > 0 && ly == TY - 1) || (dz < 0 && lz == 0);
+                const uint32_t ny = by + dy, nz = bz + dz;   // unsigned wrap -> outside
+                if (!out || ny >= g.bh || nz >= g.bd) continue;
+                const size_t nrow = ((size_t)nz * g.bh + ny) * g.bw;
+                uint32_t na, nbm, nc, ne;
+                bar_rows(M + nrow + bxs, bxs, g.bw, na, nbm, nc, ne);
+                // facing rows: ours A, theirs B (row shifts of B in a block mask)
+                const uint32_t A = q == 0 ? (a | c) : (q == 1 ? (a | b) : (q == 2 ? a : b));
+                const uint32_t B = q == 0 ? (nbm | ne) : (q == 1 ? (nc | ne) : (q == 2 ? ne : nc));
+                const uint32_t bmask = q == 0 ? 0xCCu : (q == 1 ? 0xF0u : (q == 2 ? 0xC0u : 0x30u));
+                // neighbour segments' edge columns; none across the tile's x boundary
+                const uint32_t ml = (j > 0) ? M[nrow + bxs - 1] : 0u;
+                const uint32_t mr = (j + 1 < LPB && bxs + kLaneBlocks < g.bw) ? M[nrow + bxs + kLaneBlocks] : 0u;
+                const uint32_t Bl = (ml & bmask & 0xAAu) ? 0x80000000u : 0u;   // dx = 1 voxels of the left block
+                const uint32_t Br = (mr & bmask & 0x55u) ? 1u : 0u;             // dx = 0 voxels of the right block
+                const uint32_t on = na | nbm | nc | ne;
+                SegContacts cs(A, B, Bl, Br, st, on & ~(on << 1));
+                while (cs.any()) {
+                    uint32_t x;
+                    const int p = cs.pop(x);
+                    push(P[row + bxs + (x >> 1)], P[nrow + bxs + ((p + 2) >> 1) - 1]);
+                }
+            }
+        }
+    } else {
+        const uint64_t g2 = gid - bar_jobs;
+        const uint32_t tile = (uint32_t)(g2 / X_LANES), k = (uint32_t)(g2 % X_LANES);
+        if (tile < g.ntiles) {
+            const uint32_t ly = k % TY, lz = k / TY;
+            const uint32_t bx0 = (tile % g.tiles_x) * TX, by = ((tile / g.tiles_x) % g.tiles_y) * TY + ly;
+            const uint32_t bz = (tile / (g.tiles_x * g.tiles_y)) * TZ + lz;
+            if (bx0 > 0 && bx0 < g.bw && by < g.bh && bz < g.bd) {
+                const size_t b0 = ((size_t)bz * g.bh + by) * g.bw + bx0;
+                const uint32_t m = M[b0];
+                if (m & 0x55u) {
+                    const uint32_t rb0 = P[b0];
+                    for (int e9 = 0; e9 < 9; ++e9) {
+                        const int dy = e9 % 3 - 1, dz = e9 / 3 - 1;
+                        const uint32_t ny = by + dy, nz = bz + dz;
+                        if (ny >= g.bh || nz >= g.bd) continue;
+                        const size_t n = ((size_t)nz * g.bh + ny) * g.bw + bx0 - 1;
+                        if (adjacent(m, M[n], -1, dy, dz)) push(rb0, P[n]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t n = queued < QUEUE_CAP ? queued : QUEUE_CAP;
+    for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
+        const uint2 pr = queue[i];
+        links += Forest::unite(P, pr.x, pr.y);
+    }
     if (count) count_links(slots, links);   // kernel-uniform branch
 }
 
@@ -449,7 +519,7 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     uint8_t *M = reinterpret_cast<uint8_t *>(P) + align256(nb * sizeof(uint32_t));
     const bool vec_in = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
     const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
-    const uint32_t face_grid = (uint32_t)(((uint64_t)g.ntiles * JOBS + 255) / 256);
+    const uint32_t merge_grid = (uint32_t)(((uint64_t)g.ntiles * (BAR_LANES + X_LANES) + MERGE_THREADS - 1) / MERGE_THREADS);
     const uint32_t xgroups = (bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
 
     if (kt) kt->begin(s);
@@ -461,7 +531,7 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
                            im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_tile");
-    hipLaunchKernelGGL(bleq3d_merge, dim3(face_grid), dim3(256), 0, s, P, M, g, im.count, slots);
+    hipLaunchKernelGGL(bleq3d_merge, dim3(merge_grid), dim3(MERGE_THREADS), 0, s, P, M, g, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_merge");
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
     if (vec_out)
