@@ -15,9 +15,10 @@
 //              runs of each 2-row band in LDS; writes one packed word per
 //              block, parent << 4 | mask, whose parent is the tile-local root
 //              (global block index).
-//   2. merge   one thread per block on a tile border: unions with each
-//              8-neighbour block in another tile that it touches (atomicMin
-//              links between tile-local roots).
+//   2. merge   unions across tile borders: each tile's top band against the
+//              band above (bit-mask contacts per 16-block segment) and its
+//              first pixel column against the column to its left; pairs of
+//              tile-local roots queued in LDS, united by all threads.
 //   3. flatten (optional, chosen by the autotuner) one thread per tile-border
 //              block: the block and its tile-local root are pointed straight
 //              at the global root.
@@ -60,6 +61,9 @@ __device__ __forceinline__ bool adj_upright(uint32_t c, uint32_t n) { return (c 
 
 struct One {
     Geom2D g;
+    static constexpr bool kSingle = true;
+    __device__ __forceinline__ uint32_t image_of_tile(uint32_t) const { return 0; }
+    __device__ __forceinline__ const Geom2D &image(uint32_t) const { return g; }
     __device__ __forceinline__ const Geom2D &by_tile(uint32_t gt, uint32_t &lt) const
     {
         lt = gt;
@@ -88,6 +92,9 @@ struct Many {
         }
         return lo;
     }
+    static constexpr bool kSingle = false;
+    __device__ __forceinline__ uint32_t image_of_tile(uint32_t gt) const { return find<0>(gt); }
+    __device__ __forceinline__ const Geom2D &image(uint32_t i) const { return g[i]; }
     __device__ __forceinline__ const Geom2D &by_tile(uint32_t gt, uint32_t &lt) const
     {
         const Geom2D &r = g[find<0>(gt)];
@@ -253,7 +260,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
     if (G.count) count_add(G.slots, roots);   // one image per workgroup: uniform branch
 }
 
-// ---- 2./3. tile-border kernels -----------------------------------------------------
+// ---- tile-border block jobs (the flatten kernel) ------------------------------------
 
 template <uint32_t TBW, uint32_t TBH>
 struct Border {
@@ -298,38 +305,168 @@ __device__ __forceinline__ bool border_job(const Src &src, uint32_t gid, uint32_
     return (vb & 15u) != 0;
 }
 
-// Consecutive border blocks of a tile usually repeat the same union (same
-// tile-local roots on both sides); only the first lane of such a run unites.
-template <uint32_t TBW, uint32_t TBH, class Src>
-__global__ __launch_bounds__(256) void bleq2d_merge(Src src, uint32_t total_tiles)
+// ---- 2. merge kernel ------------------------------------------------------------
+//
+// Unions across tile borders, between tile-local roots read from the block
+// words (whose mask nibbles give back the pixel rows as bit masks):
+//   * top border: one lane per 16-block segment of a tile's top band, against
+//     the band above (the last band of the tile above), with the tile
+//     kernel's contact logic;
+//   * left border: one lane per band of a tile, its first pixel column against
+//     the last column of the tile to the left, in the same band and
+//     diagonally in the bands above and below.
+// Jobs: every tile's top-border lanes, then every tile's left-border lanes.
+// Lanes collect pairs of roots in LDS; then every thread of the workgroup
+// unites pairs from there (global unions are chains of dependent loads: many
+// short chains in flight beat a few long ones).  A single image collects them
+// in a hash set, so each distinct pair is united once per workgroup (in dense
+// images every border segment of a tile yields the same pair); a batch, whose
+// pairs also carry an image, in a queue.
+
+constexpr uint32_t MERGE_THREADS = 256;
+constexpr uint32_t QUEUE_CAP = 8 * MERGE_THREADS;   // pairs per workgroup (batches)
+constexpr uint32_t HASH_CAP = 16 * MERGE_THREADS;   // hash set slots (single image)
+constexpr uint32_t HASH_PROBES = 16;
+constexpr unsigned long long kNoPair = ~0ull;
+static_assert(HASH_CAP == 4096, "the pair hash keeps 12 bits");
+
+// Block words of a 16-block segment starting at block bxs of a row (zero past bw).
+__device__ __forceinline__ void load_segment(const uint32_t *row, uint32_t bxs, uint32_t bw, uint32_t v[LANE_BLOCKS])
 {
-    const Geom2D *G = nullptr;
-    uint32_t bx = 0, by = 0, bx0 = 0, by0 = 0, vb = 0;
-    const bool valid = border_job<TBW, TBH>(src, blockIdx.x * blockDim.x + threadIdx.x, total_tiles, G, bx, by,
-                                            bx0, by0, vb);
-    // earlier neighbours: left, up-left, up, up-right; distinct (own root, neighbour root) pairs
-    uint32_t pr[4], np = 0;
-    if (valid) {
-        const uint32_t bw = G->bw, bh = G->bh, m = vb & 15u;
-        const int ndx[4] = {-1, -1, 0, 1}, ndy[4] = {0, -1, -1, -1};
+    if (bxs + LANE_BLOCKS <= bw && (reinterpret_cast<uintptr_t>(row + bxs) & 15u) == 0) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(row + bxs);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t nx = bx + ndx[k], ny = by + ndy[k];   // unsigned wrap -> out of range
-            if (nx >= bw || ny >= bh) continue;
-            if (nx - bx0 < TBW && ny - by0 < TBH) continue;    // same tile: done in the tile kernel
-            const uint32_t vn = G->P[ny * bw + nx];
-            const uint32_t nm = vn & 15u;
-            const bool adj = k == 0 ? adj_left(m, nm)
-                                    : (k == 1 ? adj_upleft(m, nm) : (k == 2 ? adj_up(m, nm) : adj_upright(m, nm)));
-            if (adj && (np == 0 || pr[np - 1] != (vn >> 4))) pr[np++] = vn >> 4;
+        for (uint32_t i = 0; i < LANE_BLOCKS / 4; ++i) {
+            const uint4 x = q[i];
+            v[4 * i] = x.x; v[4 * i + 1] = x.y; v[4 * i + 2] = x.z; v[4 * i + 3] = x.w;
+        }
+        return;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < LANE_BLOCKS; ++k) v[k] = bxs + k < bw ? row[bxs + k] : 0u;
+}
+
+// Pixel-row masks of a segment's band from its block words: top row = mask
+// bits 0-1 of each block, bottom row = bits 2-3.
+__device__ __forceinline__ void band_rows(const uint32_t v[LANE_BLOCKS], uint32_t &top, uint32_t &bot)
+{
+    top = bot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < LANE_BLOCKS; ++k) {
+        top |= (v[k] & 3u) << (2 * k);
+        bot |= ((v[k] >> 2) & 3u) << (2 * k);
+    }
+}
+
+template <uint32_t TBW, uint32_t TBH, class Src>
+__global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t total_tiles)
+{
+    constexpr uint32_t LPB = TBW / LANE_BLOCKS;
+    constexpr bool kSet = Src::kSingle;
+    __shared__ unsigned long long set[kSet ? HASH_CAP : 1];   // root a << 32 | root b
+    __shared__ uint16_t filled[kSet ? HASH_CAP : 1];          // set slots in order of insertion
+    __shared__ uint2 queue[kSet ? 1 : QUEUE_CAP];             // pairs of roots
+    __shared__ uint32_t qimg[kSet ? 1 : QUEUE_CAP];           // and their image
+    __shared__ uint32_t queued;
+    if constexpr (kSet) {
+        for (uint32_t i = threadIdx.x; i < HASH_CAP; i += MERGE_THREADS) set[i] = kNoPair;
+    }
+    if (threadIdx.x == 0) queued = 0;
+    __syncthreads();
+
+    const uint32_t gid = blockIdx.x * MERGE_THREADS + threadIdx.x, n_top = total_tiles * LPB;
+    const Geom2D *G = nullptr;
+    uint32_t img = 0, links = 0, la = ~0u, lb = ~0u;
+    const auto push = [&](uint32_t ra, uint32_t rb) {
+        if (ra == rb || (ra == la && rb == lb)) return;
+        la = ra;
+        lb = rb;
+        if constexpr (kSet) {
+            const unsigned long long key = (unsigned long long)ra << 32 | rb;
+            uint32_t h = (ra * 0x9E3779B1u ^ rb * 0x85EBCA6Bu) >> 20;   // 12 bits: HASH_CAP slots
+            for (uint32_t k = 0; k < HASH_PROBES; ++k, h = (h + 1) & (HASH_CAP - 1)) {
+                const unsigned long long old = atomicCAS(&set[h], kNoPair, key);
+                if (old == kNoPair) filled[atomicAdd(&queued, 1u)] = (uint16_t)h;   // < HASH_CAP: one per slot
+                if (old == kNoPair || old == key) return;
+            }
+        } else {
+            const uint32_t i = atomicAdd(&queued, 1u);
+            if (i < QUEUE_CAP) {
+                queue[i] = make_uint2(ra, rb);
+                qimg[i] = img;
+                return;
+            }
+        }
+        links += Forest::unite(G->P, ra, rb);   // no room: unite here
+    };
+
+    if (gid < n_top) {
+        const uint32_t gt = gid / LPB, j = gid % LPB;
+        uint32_t lt;
+        img = src.image_of_tile(gt);
+        G = &src.by_tile(gt, lt);
+        const uint32_t bw = G->bw, tx = lt % G->tiles_x, ty = lt / G->tiles_x;
+        const uint32_t by0 = ty * TBH, bxs = tx * TBW + j * LANE_BLOCKS;
+        if (ty > 0 && bxs < bw) {
+            const uint32_t *own = G->P + (size_t)by0 * bw + bxs, *up = own - bw;
+            uint32_t vo[LANE_BLOCKS], vu[LANE_BLOCKS];
+            load_segment(G->P + (size_t)by0 * bw, bxs, bw, vo);
+            load_segment(G->P + (size_t)(by0 - 1) * bw, bxs, bw, vu);
+            uint32_t a, b, au, bu;
+            band_rows(vo, a, b);
+            band_rows(vu, au, bu);
+            const uint32_t bul = bxs > 0 ? (up[-1] & 8u) << 28 : 0u;                       // bottom-right pixel -> bit 31
+            const uint32_t bur = bxs + LANE_BLOCKS < bw ? (up[LANE_BLOCKS] >> 2) & 1u : 0u;   // bottom-left pixel -> bit 0
+            const uint32_t o = a | b, ou = au | bu;
+            SegContacts cs(a, bu, bul, bur, o & ~(o << 1), ou & ~(ou << 1));
+            while (cs.any()) {
+                uint32_t x;
+                const int p = cs.pop(x);
+                push(own[x >> 1] >> 4, up[((p + 2) >> 1) - 1] >> 4);   // block of upper column p (-1 / 32: neighbours)
+            }
+        }
+    } else {
+        const uint32_t g2 = gid - n_top, gt = g2 / TBH, r = g2 % TBH;
+        if (gt < total_tiles) {
+            uint32_t lt;
+            img = src.image_of_tile(gt);
+            G = &src.by_tile(gt, lt);
+            const uint32_t bw = G->bw, bh = G->bh, tx = lt % G->tiles_x, ty = lt / G->tiles_x;
+            const uint32_t bx0 = tx * TBW, by = ty * TBH + r;
+            if (tx > 0 && by < bh) {
+                // block bits: 1 top-left, 2 top-right, 4 bottom-left, 8 bottom-right
+                const uint32_t *w = G->P + (size_t)by * bw + bx0;
+                const uint32_t vo = w[0], m = vo & 15u;
+                if ((m & 5u) && (w[-1] & 10u)) push(vo >> 4, w[-1] >> 4);   // same band
+                if (r > 0 && (m & 1u)) {                                     // band above
+                    const uint32_t vu = w[-(int64_t)bw - 1];
+                    if (vu & 8u) push(vo >> 4, vu >> 4);
+                }
+                if (r + 1 < TBH && by + 1 < bh && (m & 4u)) {                // band below
+                    const uint32_t vd = w[bw - 1];
+                    if (vd & 2u) push(vo >> 4, vd >> 4);
+                }
+            }
         }
     }
-    const uint64_t key = np ? ((uint64_t)(vb >> 4) << 32) | pr[0] : 0;
-    uint32_t leader;
-    const bool lead = run_leader(key, valid ? G->tile_base : 0u, np != 0, leader);
-    uint32_t links = 0;
-    for (uint32_t i = lead ? 0u : 1u; i < np; ++i) links += Forest::unite(G->P, vb >> 4, pr[i]);   // tile-local roots
-    count_links(valid && G->count ? G->slots : nullptr, valid && G->count ? links : 0u);
+    __syncthreads();
+    if constexpr (kSet) {
+        const Geom2D &G0 = src.image(0);
+        const uint32_t n = queued;
+        for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
+            const unsigned long long key = set[filled[i]];
+            links += Forest::unite(G0.P, (uint32_t)(key >> 32), (uint32_t)key);
+        }
+        if (G0.count) count_links(G0.slots, links);   // kernel-uniform branch
+    } else {
+        const uint32_t n = queued < QUEUE_CAP ? queued : QUEUE_CAP;
+        if (links && G->count) atomicSub(&G->slots[(blockIdx.x % kCountSlots) * kCountStride], links);
+        for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
+            const uint2 e = queue[i];
+            const Geom2D &Gi = src.image(qimg[i]);
+            if (Forest::unite(Gi.P, e.x, e.y) && Gi.count) atomicSub(&Gi.slots[(blockIdx.x % kCountSlots) * kCountStride], 1u);
+        }
+    }
 }
 
 // Point each distinct tile-local root seen on the tile border (and the border
@@ -464,10 +601,12 @@ struct Variant {
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
                              KernelTimer *kt, bool flatten)
     {
+        const uint32_t merge_grid =
+            (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + MERGE_THREADS - 1) / MERGE_THREADS);
         const uint32_t border_grid = (uint32_t)(((uint64_t)total_tiles * Border<TBW, TBH>::JOBS + 255) / 256);
         hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src>), dim3(total_tiles), dim3(THREADS), 0, s, src);
         if (kt) kt->mark(s, "bleq2d_tile");
-        hipLaunchKernelGGL((bleq2d_merge<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
+        hipLaunchKernelGGL((bleq2d_merge<TBW, TBH, Src>), dim3(merge_grid), dim3(MERGE_THREADS), 0, s, src, total_tiles);
         if (kt) kt->mark(s, "bleq2d_merge");
         if (flatten) {
             hipLaunchKernelGGL((bleq2d_flatten<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
