@@ -18,7 +18,7 @@
 //   2. merge   unions across tile borders: each tile's top band against the
 //              band above (bit-mask contacts per 16-block segment) and its
 //              first pixel column against the column to its left; pairs of
-//              tile-local roots queued in LDS, united by all threads.
+//              tile-local roots collected in LDS, united by all threads.
 //   3. flatten (optional, chosen by the autotuner) one thread per tile-border
 //              block: the block and its tile-local root are pointed straight
 //              at the global root.
@@ -325,10 +325,7 @@ __device__ __forceinline__ bool border_job(const Src &src, uint32_t gid, uint32_
 
 constexpr uint32_t MERGE_THREADS = 256;
 constexpr uint32_t QUEUE_CAP = 8 * MERGE_THREADS;   // pairs per workgroup (batches)
-constexpr uint32_t HASH_CAP = 16 * MERGE_THREADS;   // hash set slots (single image)
-constexpr uint32_t HASH_PROBES = 16;
-constexpr unsigned long long kNoPair = ~0ull;
-static_assert(HASH_CAP == 4096, "the pair hash keeps 12 bits");
+constexpr uint32_t HASH_CAP = 16 * MERGE_THREADS;   // pair set slots (single image)
 
 // Block words of a 16-block segment starting at block bxs of a row (zero past bw).
 __device__ __forceinline__ void load_segment(const uint32_t *row, uint32_t bxs, uint32_t bw, uint32_t v[LANE_BLOCKS])
@@ -363,14 +360,11 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
 {
     constexpr uint32_t LPB = TBW / LANE_BLOCKS;
     constexpr bool kSet = Src::kSingle;
-    __shared__ unsigned long long set[kSet ? HASH_CAP : 1];   // root a << 32 | root b
-    __shared__ uint16_t filled[kSet ? HASH_CAP : 1];          // set slots in order of insertion
-    __shared__ uint2 queue[kSet ? 1 : QUEUE_CAP];             // pairs of roots
-    __shared__ uint32_t qimg[kSet ? 1 : QUEUE_CAP];           // and their image
+    __shared__ LdsPairSet<kSet ? HASH_CAP : 64> set;   // pairs of roots (single image)
+    __shared__ uint2 queue[kSet ? 1 : QUEUE_CAP];      // pairs of roots (batches)
+    __shared__ uint32_t qimg[kSet ? 1 : QUEUE_CAP];    // and their image
     __shared__ uint32_t queued;
-    if constexpr (kSet) {
-        for (uint32_t i = threadIdx.x; i < HASH_CAP; i += MERGE_THREADS) set[i] = kNoPair;
-    }
+    if constexpr (kSet) set.init();
     if (threadIdx.x == 0) queued = 0;
     __syncthreads();
 
@@ -382,13 +376,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
         la = ra;
         lb = rb;
         if constexpr (kSet) {
-            const unsigned long long key = (unsigned long long)ra << 32 | rb;
-            uint32_t h = (ra * 0x9E3779B1u ^ rb * 0x85EBCA6Bu) >> 20;   // 12 bits: HASH_CAP slots
-            for (uint32_t k = 0; k < HASH_PROBES; ++k, h = (h + 1) & (HASH_CAP - 1)) {
-                const unsigned long long old = atomicCAS(&set[h], kNoPair, key);
-                if (old == kNoPair) filled[atomicAdd(&queued, 1u)] = (uint16_t)h;   // < HASH_CAP: one per slot
-                if (old == kNoPair || old == key) return;
-            }
+            if (set.insert(ra, rb)) return;
         } else {
             const uint32_t i = atomicAdd(&queued, 1u);
             if (i < QUEUE_CAP) {
@@ -452,10 +440,10 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
     __syncthreads();
     if constexpr (kSet) {
         const Geom2D &G0 = src.image(0);
-        const uint32_t n = queued;
+        const uint32_t n = set.size();
         for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
-            const unsigned long long key = set[filled[i]];
-            links += Forest::unite(G0.P, (uint32_t)(key >> 32), (uint32_t)key);
+            const uint2 pr = set.pair(i);
+            links += Forest::unite(G0.P, pr.x, pr.y);
         }
         if (G0.count) count_links(G0.slots, links);   // kernel-uniform branch
     } else {

@@ -19,7 +19,8 @@
 //   2. merge   unions across tile faces: bar segments against their earlier
 //              neighbour bars in other tiles (bit-mask contacts), and the
 //              tile's first block column against the column to its left;
-//              pairs of tile-local roots queued in LDS, united by all threads.
+//              distinct pairs of tile-local roots collected in LDS, united
+//              by all threads.
 //   3. final   one thread per 2 blocks: root of each block (neighbouring lanes
 //              share one chase per tile-local root), writes the 16 voxel
 //              labels of the 2 blocks, counts roots.
@@ -281,16 +282,17 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
 //     the contacts (x-boundary contacts excepted: the next kind has them);
 //   * x lanes: one per bar at the tile's first block column, against the nine
 //     blocks at the column to the left.
-// Lanes push distinct pairs of roots into a workgroup queue in LDS, then every
-// thread of the workgroup unites pairs from the queue (a lane with many
-// contacts does not serialize its unions).
+// Lanes put pairs of roots into a workgroup hash set in LDS, then every thread
+// of the workgroup unites the distinct pairs (a lane with many contacts does
+// not serialize its unions, and the many contacts of two large components
+// across a face become one union).
 
 struct Grid3 {
     uint32_t bd, bh, bw, tiles_x, tiles_y, ntiles;
 };
 
 constexpr uint32_t MERGE_THREADS = 256;
-constexpr uint32_t QUEUE_CAP = 8 * MERGE_THREADS;   // pairs per workgroup
+constexpr uint32_t PAIR_CAP = 16 * MERGE_THREADS;   // pair set slots per workgroup
 
 // The four voxel rows of 4 blocks (mask bit dx + 2 dy + 4 dz) as 8 bits each:
 // bit 2k + dx of row (dy, dz) for block k.
@@ -331,9 +333,8 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
                                                               uint32_t *count, uint32_t *slots)
 {
     constexpr uint32_t LPB = TX / kLaneBlocks;
-    __shared__ uint2 queue[QUEUE_CAP];
-    __shared__ uint32_t queued;
-    if (threadIdx.x == 0) queued = 0;
+    __shared__ LdsPairSet<PAIR_CAP> set;   // distinct pairs of roots
+    set.init();
     __syncthreads();
 
     const uint64_t gid = (uint64_t)blockIdx.x * MERGE_THREADS + threadIdx.x;
@@ -344,9 +345,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
         if (ra == rb || (ra == la && rb == lb)) return;
         la = ra;
         lb = rb;
-        const uint32_t i = atomicAdd(&queued, 1u);
-        if (i < QUEUE_CAP) queue[i] = make_uint2(ra, rb);
-        else links += Forest::unite(P, ra, rb);   // queue full: unite here
+        if (!set.insert(ra, rb)) links += Forest::unite(P, ra, rb);   // no room: unite here
     };
 
     if (gid < bar_jobs) {
@@ -411,9 +410,9 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
         }
     }
     __syncthreads();
-    const uint32_t n = queued < QUEUE_CAP ? queued : QUEUE_CAP;
+    const uint32_t n = set.size();
     for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
-        const uint2 pr = queue[i];
+        const uint2 pr = set.pair(i);
         links += Forest::unite(P, pr.x, pr.y);
     }
     if (count) count_links(slots, links);   // kernel-uniform branch

@@ -311,6 +311,47 @@ __device__ __forceinline__ void count_clear(uint32_t *slots)
     if (threadIdx.x < kCountSlots) slots[threadIdx.x * kCountStride] = 0;
 }
 
+// ---- distinct pairs, collected in LDS by a workgroup ------------------------------
+//
+// An open-addressing hash set of (a, b) pairs (a, b < 2^32 - 1) plus the list of
+// its filled slots in insertion order, so that a workgroup can unite each
+// distinct pair once, spread evenly over its threads:
+//   init() by every thread, __syncthreads(), insert() by any thread,
+//   __syncthreads(), then pairs 0..size()-1.
+// insert() returns false when its probe sequence finds no room: the caller
+// handles that pair itself.
+template <uint32_t CAP, uint32_t PROBES = 16>
+struct LdsPairSet {
+    static_assert((CAP & (CAP - 1)) == 0 && CAP >= 64 && CAP <= 65536, "CAP: power of two, 64..65536");
+    static constexpr unsigned long long kEmpty = ~0ull;
+    unsigned long long key[CAP];
+    uint16_t filled[CAP];
+    uint32_t n;
+
+    __device__ __forceinline__ void init()
+    {
+        for (uint32_t i = threadIdx.x; i < CAP; i += blockDim.x) key[i] = kEmpty;
+        if (threadIdx.x == 0) n = 0;
+    }
+    __device__ __forceinline__ bool insert(uint32_t a, uint32_t b)
+    {
+        const unsigned long long k = (unsigned long long)a << 32 | b;
+        uint32_t h = (a * 0x9E3779B1u ^ b * 0x85EBCA6Bu) >> (32 - __builtin_ctz(CAP));
+        for (uint32_t t = 0; t < PROBES; ++t, h = (h + 1) & (CAP - 1)) {
+            const unsigned long long old = atomicCAS(&key[h], kEmpty, k);
+            if (old == kEmpty) filled[atomicAdd(&n, 1u)] = (uint16_t)h;   // one entry per slot: < CAP
+            if (old == kEmpty || old == k) return true;
+        }
+        return false;
+    }
+    __device__ __forceinline__ uint32_t size() const { return n; }
+    __device__ __forceinline__ uint2 pair(uint32_t i) const
+    {
+        const unsigned long long k = key[filled[i]];
+        return make_uint2((uint32_t)(k >> 32), (uint32_t)k);
+    }
+};
+
 // ---- bit masks of pixel rows ------------------------------------------------
 
 // bit i set <=> byte i of w is non-zero (i = 0..3)
