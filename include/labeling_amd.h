@@ -62,8 +62,12 @@ typedef enum {
      * the block grid of the image zero-padded to ((n>>5)<<5)+32
      * (LabelingAlgs.cl:202,317; LabelingTools.cpp:103). */
     LBL_LABELS_REFERENCE = 1,
-    /* 1..N in raster order of each component's first pixel (the canonical form
-     * used for parity).  Costs one extra pass. */
+    /* 1..N in raster order of each component's first pixel (voxel), x
+     * fastest: the canonical form used for parity, computed on the device.
+     * Single images and volumes, both algorithms (lbl_label2d/3d[_dev],
+     * lbl_label2d_each); batches and serving return LBL_EUNSUPPORTED.  Costs a
+     * key pass over the block words plus a bit-array scan (bleq), or a relabel
+     * pass over the labels (lbeq), and a larger workspace. */
     LBL_LABELS_CANONICAL = 2
 } lbl_label_mode;
 

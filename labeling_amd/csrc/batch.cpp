@@ -347,6 +347,7 @@ extern "C" lbl_status lbl_autotune2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d
     }
     Image2D im{d_px, pitch_bytes, h, w, d_labels, ctx->d_misc};
     Options opt = ctx->opt;
+    opt.mode = LBL_LABELS_ROOT;   // the label mode adds passes outside the tuned launches
     float best = -1.f;
     TuneVal bestv{0, true, 0.f};
     for (uint32_t t = 0; t < kBleq2dTileCount; ++t)

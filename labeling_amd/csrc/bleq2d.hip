@@ -243,7 +243,9 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
         for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {
             if (st & (3u << (2 * kk))) {
                 cur = (v[kk] & kResolved) ? v[kk] : g0 + kk;
-                roots += (cur & ~kResolved) == g0 + kk;
+                const bool root = (cur & ~kResolved) == g0 + kk;
+                roots += root;
+                if (root && (G.flags & kCanon)) G.canon.key[g0 + kk] = ~0u;   // global roots are tile-local roots
             }
             const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
             out[kk] = m ? (cur << 4) | m : (g0 + kk) << 4;   // << 4 drops the tag bit
@@ -508,17 +510,18 @@ __device__ __forceinline__ void store_row4(uint32_t *dst, const uint32_t v[4], u
     for (uint32_t i = 0; i < 4 && x0 + i < w; ++i) dst[i] = v[i];
 }
 
-template <class Src>
-__global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
+// Block words of one lane's FINAL_BLOCKS blocks (zero outside the image) and
+// the blocks' roots (where the mask is not empty).  Neighbouring lanes mostly
+// share their blocks' parent (a tile-local root): the first lane of each run of
+// lanes with the same first parent chases it, the others take its root.
+__device__ __forceinline__ void lane_blocks(const Geom2D &G, uint32_t by, uint32_t bxs, uint32_t v[FINAL_BLOCKS],
+                                            uint32_t root[FINAL_BLOCKS])
 {
-    uint32_t lw;
-    const Geom2D &G = src.by_final(blockIdx.x, lw);
-    const uint32_t h = G.h, w = G.w, bw = G.bw, bh = G.bh;
-    const uint32_t by = lw / G.xgroups;
-    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    const uint32_t bw = G.bw;
     const uint32_t *__restrict__ P = G.P;
-    const bool inside = by < bh && bxs < bw;
-    uint32_t v[FINAL_BLOCKS] = {};
+    const bool inside = by < G.bh && bxs < bw;
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = root[k] = 0;
     if (inside) {
         const uint32_t *src_w = P + (size_t)by * bw + bxs;
         if ((reinterpret_cast<uintptr_t>(src_w) & 7u) == 0 && bxs + FINAL_BLOCKS <= bw) {
@@ -528,9 +531,6 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
             for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = bxs + k < bw ? src_w[k] : 0u;
         }
     }
-    // Neighbouring lanes mostly share their blocks' parent (a tile-local root):
-    // the first lane of each run of lanes with the same first parent chases
-    // it, the others take its root.
     uint32_t key = 0;
     bool has = false;
 #pragma unroll
@@ -543,21 +543,46 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
     const bool lead = run_leader(key, 0u, has, leader);
     uint32_t key_root = lead ? Forest::find_final(P, key) : 0u;
     key_root = __shfl(key_root, (int)leader, 64);
-
     if (inside) {
-        uint32_t lab[FINAL_BLOCKS];
         uint32_t prev_parent = key, prev_root = key_root;
-        const bool ref = (G.flags & kRefLabels) != 0;
+#pragma unroll
+        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+            if (!(v[k] & 15u)) continue;
+            const uint32_t p = v[k] >> 4;
+            root[k] = p == prev_parent ? prev_root : Forest::find_final(P, p);
+            prev_parent = p;
+            prev_root = root[k];
+        }
+    }
+}
+
+template <class Src>
+__global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
+{
+    uint32_t lw;
+    const Geom2D &G = src.by_final(blockIdx.x, lw);
+    const uint32_t h = G.h, w = G.w, bw = G.bw;
+    const uint32_t by = lw / G.xgroups;
+    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    uint32_t v[FINAL_BLOCKS], root[FINAL_BLOCKS];
+    lane_blocks(G, by, bxs, v, root);
+
+    if (by < G.bh && bxs < bw) {
+        uint32_t lab[FINAL_BLOCKS];
+        const bool ref = (G.flags & kRefLabels) != 0, canon = (G.flags & kCanon) != 0;
+        uint32_t prev_root = ~0u, prev_lab = 0;
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
             lab[k] = 0;
             if (!(v[k] & 15u)) continue;
-            const uint32_t p = v[k] >> 4;
-            const uint32_t root = p == prev_parent ? prev_root : Forest::find_final(P, p);
-            prev_parent = p;
-            prev_root = root;
-            // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
-            lab[k] = ref ? 1u + (root % bw) + (root / bw) * G.ref_bw : root + 1u;
+            const uint32_t r = root[k];
+            if (r != prev_root) {
+                prev_root = r;
+                // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
+                prev_lab = canon ? canon_label(G.canon, G.canon.key[r])
+                                 : (ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u);
+            }
+            lab[k] = prev_lab;
         }
         const uint32_t y0 = 2 * by, x0 = 2 * bxs;
         uint32_t *row0 = G.labels + (size_t)y0 * w + x0;
@@ -579,6 +604,82 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
     if (lw == 0 && G.count && threadIdx.x < kCountSlots) count_reduce(G.slots, G.count);
 }
 
+// ---- canonical numbering (kCanon): keys, then bits at the keys -----------------------
+//
+// A component's first pixel lies in its topmost block row, which is the block
+// row of its root (the minimum block index); each block of that row in the
+// component offers its own first pixel, and the smallest offer is the
+// component's key: first-pixel row * bw + block column.  The keys kernel also
+// clears the key bits; the mark kernel then sets the bit of every root's key.
+
+template <class Src>
+__global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
+{
+    uint32_t lw;
+    const Geom2D &G = src.by_final(blockIdx.x, lw);
+    const Canon &c = G.canon;
+    const uint64_t t = (uint64_t)lw * NT_FINAL + threadIdx.x;   // covers every word: bw * h / 64 < bw * bh * 2 / 2
+    if (t < c.nwords) c.bits[t] = 0ull;
+
+    const uint32_t bw = G.bw;
+    const uint32_t by = lw / G.xgroups;
+    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    uint32_t v[FINAL_BLOCKS], root[FINAL_BLOCKS];
+    lane_blocks(G, by, bxs, v, root);
+
+    // this lane's offers: (root, key) of its first candidate block, and of a
+    // second one with another root
+    const uint32_t row0 = by * bw;
+    uint32_t r0 = ~0u, k0 = ~0u, r1 = ~0u, k1 = ~0u;
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+        const uint32_t m = v[k] & 15u;
+        if (!m || root[k] - row0 >= bw) continue;   // not in its component's top block row
+        const uint32_t key = (2 * by + ((m & 3u) ? 0u : 1u)) * bw + bxs + k;
+        if (r0 == ~0u || r0 == root[k]) {
+            r0 = root[k];
+            k0 = min(k0, key);
+        } else {
+            r1 = root[k];
+            k1 = key;
+        }
+    }
+    // one atomic per wave when every offer of the wave is for one root
+    const bool has = r0 != ~0u;
+    const uint64_t any = __ballot(has);
+    if (!any) return;
+    const uint32_t R = __shfl(r0, (int)__builtin_ctzll(any), 64);
+    if (__ballot(has && (r0 != R || r1 != ~0u)) == 0) {
+        uint32_t kmin = k0;
+        for (int off = 32; off > 0; off >>= 1) kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off, 64));
+        if ((threadIdx.x & 63u) == (uint32_t)__builtin_ctzll(any)) atomicMin(&c.key[R], kmin);
+    } else {
+        if (has) atomicMin(&c.key[r0], k0);
+        if (r1 != ~0u) atomicMin(&c.key[r1], k1);
+    }
+}
+
+template <class Src>
+__global__ __launch_bounds__(NT_FINAL) void bleq2d_mark(Src src)
+{
+    uint32_t lw;
+    const Geom2D &G = src.by_final(blockIdx.x, lw);
+    const uint32_t bw = G.bw;
+    const uint32_t by = lw / G.xgroups;
+    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    if (by >= G.bh) return;
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+        const uint32_t b = by * bw + bxs + k;
+        if (bxs + k >= bw) break;
+        const uint32_t vb = G.P[b];
+        if ((vb & 15u) && (vb >> 4) == b) {   // a root
+            const uint32_t key = G.canon.key[b];
+            if ((key >> 6) < G.canon.nwords) atomicOr(&G.canon.bits[key >> 6], 1ull << (key & 63u));
+        }
+    }
+}
+
 // ---- host side --------------------------------------------------------------------
 
 template <uint32_t TBW, uint32_t TBH>
@@ -587,7 +688,7 @@ struct Variant {
 
     template <class Src>
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
-                             KernelTimer *kt, bool flatten)
+                             KernelTimer *kt, bool flatten, const Canon *canon)
     {
         const uint32_t merge_grid =
             (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + MERGE_THREADS - 1) / MERGE_THREADS);
@@ -600,24 +701,33 @@ struct Variant {
             hipLaunchKernelGGL((bleq2d_flatten<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
             if (kt) kt->mark(s, "bleq2d_flatten");
         }
+        if (canon) {
+            hipLaunchKernelGGL((bleq2d_keys<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+            if (kt) kt->mark(s, "bleq2d_keys");
+            hipLaunchKernelGGL((bleq2d_mark<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+            if (kt) kt->mark(s, "bleq2d_mark");
+            const hipError_t e = canon_scan_launch(*canon, s, kt);
+            if (e != hipSuccess) return e;
+        }
         hipLaunchKernelGGL((bleq2d_final<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
         if (kt) kt->mark(s, "bleq2d_final");
         return hipGetLastError();
     }
 };
 
+// canon: the single image's canonical state when it asks for canonical labels
 template <class Src>
 hipError_t dispatch(uint32_t tile_id, const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
-                    KernelTimer *kt, bool flatten)
+                    KernelTimer *kt, bool flatten, const Canon *canon = nullptr)
 {
     switch (tile_id) {
-    case 1: return Variant<64, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 2: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 3: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 4: return Variant<256, 64>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 5: return Variant<512, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
-    case 6: return Variant<128, 128>::launch(src, total_tiles, total_final, s, kt, flatten);
-    default: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten);
+    case 1: return Variant<64, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
+    case 2: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
+    case 3: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
+    case 4: return Variant<256, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
+    case 5: return Variant<512, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
+    case 6: return Variant<128, 128>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
+    default: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
     }
 }
 
@@ -636,15 +746,16 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
     g.xgroups = (g.bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
     g.nfinal = g.xgroups * g.bh;
     g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
-    g.flags &= kRefLabels;
+    g.flags &= kRefLabels | kCanon;
     if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
     if (g.w % 8 == 0 && reinterpret_cast<uintptr_t>(g.labels) % 16 == 0) g.flags |= kVecOut;
 }
 
-size_t bleq2d_workspace(uint32_t h, uint32_t w)
+size_t bleq2d_workspace(uint32_t h, uint32_t w, bool canonical)
 {
     const uint64_t bw = (w + 1ull) / 2, bh = (h + 1ull) / 2;
-    return align256(kCountBytes) + align256(bw * bh * sizeof(uint32_t));
+    return align256(kCountBytes) + align256(bw * bh * sizeof(uint32_t)) +
+           (canonical ? canon_workspace((uint64_t)h * bw, bw * bh) : 0);
 }
 
 hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
@@ -660,10 +771,13 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
     g.count = im.count;
     g.slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
     g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
-    g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
+    g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : (opt.mode == LBL_LABELS_CANONICAL ? kCanon : 0u);
     bleq2d_geometry(g, opt.tile2d);
+    if (g.flags & kCanon)
+        g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256((size_t)g.bw * g.bh * sizeof(uint32_t)),
+                               (uint64_t)g.h * g.bw, (uint64_t)g.bw * g.bh);
     if (kt) kt->begin(s);
-    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten);
+    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, (g.flags & kCanon) ? &g.canon : nullptr);
 }
 
 hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,

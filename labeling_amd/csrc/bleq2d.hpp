@@ -2,7 +2,7 @@
 // the single-image entry points and the batch planner.
 #pragma once
 
-#include "lbl_internal.hpp"
+#include "canon.hpp"
 
 namespace lbl {
 
@@ -10,6 +10,7 @@ enum : uint32_t {
     kVecIn = 1u,       // image rows start 16-byte aligned (pitch and base)
     kVecOut = 2u,      // label rows start 16-byte aligned (w % 8 == 0, base)
     kRefLabels = 4u,   // write the reference's raw label values
+    kCanon = 8u,       // write canonical labels (canon.hpp)
 };
 
 // One image as the kernels see it (also the element of a batch's device table).
@@ -27,6 +28,7 @@ struct Geom2D {
     uint32_t final_base;        // first global final workgroup id (batch)
     uint32_t flags;
     uint32_t ref_bw;            // block-grid width of the reference's padded image
+    Canon canon;                // canonical numbering state (kCanon)
 };
 
 struct TileShape2D {
@@ -37,7 +39,7 @@ constexpr uint32_t kBleq2dTileCount = 7;
 extern const TileShape2D kBleq2dTiles[kBleq2dTileCount];
 
 // Fill bw..nfinal and the alignment flags for a tile shape (px, labels, pitch,
-// h, w and the kRefLabels flag must be set).
+// h, w and the kRefLabels / kCanon flags must be set).
 void bleq2d_geometry(Geom2D &g, uint32_t tile_id);
 
 hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,

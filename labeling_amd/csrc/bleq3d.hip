@@ -27,6 +27,7 @@
 //
 // Block mask bit of voxel (dx, dy, dz) in {0,1}^3 is dx + 2 dy + 4 dz.
 #include "lbl_internal.hpp"
+#include "canon.hpp"
 #include "uf_device.hpp"
 
 namespace lbl {
@@ -111,7 +112,7 @@ template <uint32_t TBW, uint32_t TBY, uint32_t TBZ, bool VEC>
 __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     const uint8_t *__restrict__ vox, uint32_t d, uint32_t h, uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
     uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P, uint8_t *__restrict__ M, uint32_t *count,
-    uint32_t *slots)
+    uint32_t *slots, uint32_t *ckey)
 {
     using T = Tile3<TBW, TBY, TBZ>;
     constexpr uint32_t LPB = T::LPB;
@@ -248,6 +249,7 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
             if (st & (3u << (2 * k))) {
                 cur = (v[k] & kResolved) ? (v[k] & ~kResolved) : g0 + k;
                 roots += cur == g0 + k;
+                if (ckey && cur == g0 + k) ckey[g0 + k] = ~0u;   // canonical keys: global roots are tile-local roots
             }
             const uint32_t m = ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
                                (((e >> (2 * k)) & 3u) << 6);
@@ -420,18 +422,19 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
 
 // ---- 3. final kernel --------------------------------------------------------------
 
+// Parents (P) and masks (M) of one lane's FINAL_BLOCKS blocks (zero outside
+// the volume) and the blocks' roots (where the mask is not empty).
+// Neighbouring lanes mostly share their blocks' parent (a tile-local root):
+// the first lane of each run of lanes with the same first parent chases it,
+// the others take its root.
 template <bool VEC>
-__global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
-                                                          uint32_t *__restrict__ labels, uint32_t d, uint32_t h,
-                                                          uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
-                                                          uint32_t xgroups, uint32_t *count, uint32_t *slots)
+__device__ __forceinline__ void lane_blocks3(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
+                                             size_t b0, bool inside, uint32_t bxs, uint32_t bw,
+                                             uint32_t pv[FINAL_BLOCKS], uint32_t mv[FINAL_BLOCKS],
+                                             uint32_t root[FINAL_BLOCKS])
 {
-    const uint32_t row = blockIdx.x / xgroups;           // block row index bz * bh + by
-    const uint32_t by = row % bh, bz = row / bh;
-    const uint32_t bxs = ((blockIdx.x % xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
-    const bool inside = bz < bd && bxs < bw;
-    const size_t b0 = ((size_t)bz * bh + by) * bw + bxs;
-    uint32_t pv[FINAL_BLOCKS] = {}, mv[FINAL_BLOCKS] = {};
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) pv[k] = mv[k] = root[k] = 0;
     if (inside) {
         if (VEC && bxs + FINAL_BLOCKS <= bw) {
             const uint2 q = *reinterpret_cast<const uint2 *>(P + b0);
@@ -445,9 +448,6 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
             }
         }
     }
-    // Neighbouring lanes mostly share their blocks' parent (a tile-local root):
-    // the first lane of each run of lanes with the same first parent chases
-    // it, the others take its root.
     uint32_t key = 0;
     bool has = false;
 #pragma unroll
@@ -460,18 +460,45 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
     const bool lead = run_leader(key, 0u, has, leader);
     uint32_t key_root = lead ? Forest::find_final(P, key) : 0u;
     key_root = __shfl(key_root, (int)leader, 64);
+    if (inside) {
+        uint32_t prev_parent = key, prev_root = key_root;
+#pragma unroll
+        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+            if (!mv[k]) continue;
+            root[k] = pv[k] == prev_parent ? prev_root : Forest::find_final(P, pv[k]);
+            prev_parent = pv[k];
+            prev_root = root[k];
+        }
+    }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
+                                                          uint32_t *__restrict__ labels, uint32_t d, uint32_t h,
+                                                          uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
+                                                          uint32_t xgroups, uint32_t *count, uint32_t *slots,
+                                                          Canon canon, bool canonical)
+{
+    const uint32_t row = blockIdx.x / xgroups;           // block row index bz * bh + by
+    const uint32_t by = row % bh, bz = row / bh;
+    const uint32_t bxs = ((blockIdx.x % xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    const bool inside = bz < bd && bxs < bw;
+    const size_t b0 = ((size_t)bz * bh + by) * bw + bxs;
+    uint32_t pv[FINAL_BLOCKS], mv[FINAL_BLOCKS], root[FINAL_BLOCKS];
+    lane_blocks3<VEC>(P, M, b0, inside, bxs, bw, pv, mv, root);
 
     if (inside) {
         uint32_t lab[FINAL_BLOCKS];
-        uint32_t prev_parent = key, prev_root = key_root;
+        uint32_t prev_root = ~0u, prev_lab = 0;
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
             lab[k] = 0;
             if (!mv[k]) continue;
-            const uint32_t root = pv[k] == prev_parent ? prev_root : Forest::find_final(P, pv[k]);
-            prev_parent = pv[k];
-            prev_root = root;
-            lab[k] = root + 1u;
+            if (root[k] != prev_root) {
+                prev_root = root[k];
+                prev_lab = canonical ? canon_label(canon, canon.key[root[k]]) : root[k] + 1u;
+            }
+            lab[k] = prev_lab;
         }
         const uint32_t x0 = 2 * bxs;
 #pragma unroll
@@ -498,15 +525,81 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
     if (blockIdx.x == 0 && count && threadIdx.x < kCountSlots) count_reduce(slots, count);
 }
 
-}  // namespace
+// ---- canonical numbering: keys, then bits at the keys -----------------------------
+//
+// A component's first voxel ([z][y][x] raster order) lies in its front block
+// plane, the plane of its root (the minimum block index); each block of that
+// plane in the component offers its own first voxel (the lowest mask bit, as
+// bit dx + 2 dy + 4 dz orders the block's voxels), and the smallest offer is
+// the component's key: (z * h + y) * bw + block column.  The keys kernel also
+// clears the key bits; the mark kernel then sets the bit of every root's key.
 
-size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w)
+template <bool VEC>
+__global__ __launch_bounds__(NT_FINAL) void bleq3d_keys(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
+                                                         uint32_t h, uint32_t bd, uint32_t bh, uint32_t bw,
+                                                         uint32_t xgroups, Canon c)
 {
-    const uint64_t nb = ((d + 1ull) / 2) * ((h + 1ull) / 2) * ((w + 1ull) / 2);
-    return align256(kCountBytes) + align256(nb * sizeof(uint32_t)) + align256(nb);
+    const uint64_t t = (uint64_t)blockIdx.x * NT_FINAL + threadIdx.x;   // covers every word
+    if (t < c.nwords) c.bits[t] = 0ull;
+    const uint32_t row = blockIdx.x / xgroups;
+    const uint32_t by = row % bh, bz = row / bh;
+    const uint32_t bxs = ((blockIdx.x % xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
+    const bool inside = bz < bd && bxs < bw;
+    const size_t b0 = ((size_t)bz * bh + by) * bw + bxs;
+    uint32_t pv[FINAL_BLOCKS], mv[FINAL_BLOCKS], root[FINAL_BLOCKS];
+    lane_blocks3<VEC>(P, M, b0, inside, bxs, bw, pv, mv, root);
+
+    const uint32_t plane = bh * bw, plane0 = bz * plane;
+    uint32_t r0 = ~0u, k0 = ~0u, r1 = ~0u, k1 = ~0u;
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+        const uint32_t m = mv[k];
+        if (!m || root[k] - plane0 >= plane) continue;   // not in its component's front plane
+        const uint32_t bit = __builtin_ctz(m), dy = (bit >> 1) & 1u, dz = bit >> 2;
+        const uint32_t key = ((2 * bz + dz) * h + 2 * by + dy) * bw + bxs + k;
+        if (r0 == ~0u || r0 == root[k]) {
+            r0 = root[k];
+            k0 = min(k0, key);
+        } else {
+            r1 = root[k];
+            k1 = key;
+        }
+    }
+    const bool has = r0 != ~0u;
+    const uint64_t any = __ballot(has);
+    if (!any) return;
+    const uint32_t R = __shfl(r0, (int)__builtin_ctzll(any), 64);
+    if (__ballot(has && (r0 != R || r1 != ~0u)) == 0) {   // one root for the whole wave: one atomic
+        uint32_t kmin = k0;
+        for (int off = 32; off > 0; off >>= 1) kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off, 64));
+        if ((threadIdx.x & 63u) == (uint32_t)__builtin_ctzll(any)) atomicMin(&c.key[R], kmin);
+    } else {
+        if (has) atomicMin(&c.key[r0], k0);
+        if (r1 != ~0u) atomicMin(&c.key[r1], k1);
+    }
 }
 
-hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t s, KernelTimer *kt)
+__global__ __launch_bounds__(256) void bleq3d_mark(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
+                                                   uint64_t nb, Canon c)
+{
+    const uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b >= nb) return;
+    if (M[b] && P[b] == b) {   // a root
+        const uint32_t key = c.key[b];
+        if ((key >> 6) < c.nwords) atomicOr(&c.bits[key >> 6], 1ull << (key & 63u));
+    }
+}
+
+}  // namespace
+
+size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical)
+{
+    const uint64_t nb = ((d + 1ull) / 2) * ((h + 1ull) / 2) * ((w + 1ull) / 2);
+    return align256(kCountBytes) + align256(nb * sizeof(uint32_t)) + align256(nb) +
+           (canonical ? canon_workspace((uint64_t)d * h * ((w + 1ull) / 2), nb) : 0);
+}
+
+hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
 {
     const uint32_t bd = (im.d + 1u) / 2, bh = (im.h + 1u) / 2, bw = (im.w + 1u) / 2;
     const uint64_t nb = (uint64_t)bd * bh * bw;
@@ -516,6 +609,8 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     uint32_t *slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
     uint32_t *P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
     uint8_t *M = reinterpret_cast<uint8_t *>(P) + align256(nb * sizeof(uint32_t));
+    const bool canonical = opt.mode == LBL_LABELS_CANONICAL;
+    const Canon canon = canonical ? canon_layout(M + align256(nb), (uint64_t)im.d * im.h * bw, nb) : Canon{};
     const bool vec_in = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
     const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
     const uint32_t merge_grid = (uint32_t)(((uint64_t)g.ntiles * (BAR_LANES + X_LANES) + MERGE_THREADS - 1) / MERGE_THREADS);
@@ -525,20 +620,29 @@ hipError_t bleq3d_run(const Image3D &im, const Options &, void *ws, hipStream_t 
     constexpr uint32_t NT_TILE = Tile3<TX, TY, TZ>::THREADS;
     if (vec_in)
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
-                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
+                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key);
     else
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
-                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots);
+                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key);
     if (kt) kt->mark(s, "bleq3d_tile");
     hipLaunchKernelGGL(bleq3d_merge, dim3(merge_grid), dim3(MERGE_THREADS), 0, s, P, M, g, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_merge");
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
+    if (canonical) {
+        if (vec_out) hipLaunchKernelGGL(bleq3d_keys<true>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.h, bd, bh, bw, xgroups, canon);
+        else hipLaunchKernelGGL(bleq3d_keys<false>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.h, bd, bh, bw, xgroups, canon);
+        if (kt) kt->mark(s, "bleq3d_keys");
+        hipLaunchKernelGGL(bleq3d_mark, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, P, M, nb, canon);
+        if (kt) kt->mark(s, "bleq3d_mark");
+        const hipError_t e = canon_scan_launch(canon, s, kt);
+        if (e != hipSuccess) return e;
+    }
     if (vec_out)
         hipLaunchKernelGGL(bleq3d_final<true>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd, bh,
-                           bw, xgroups, im.count, slots);
+                           bw, xgroups, im.count, slots, canon, canonical);
     else
         hipLaunchKernelGGL(bleq3d_final<false>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd,
-                           bh, bw, xgroups, im.count, slots);
+                           bh, bw, xgroups, im.count, slots, canon, canonical);
     if (kt) kt->mark(s, "bleq3d_final");
     return hipGetLastError();
 }

@@ -105,8 +105,6 @@ lbl_status check_mode(lbl_ctx *ctx, lbl_alg alg, bool is3d)
     const lbl_label_mode m = ctx->opt.mode;
     if (m == LBL_LABELS_REFERENCE && (alg != LBL_BLEQ || is3d))
         return fail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_REFERENCE is defined for 2D bleq only");
-    if (m == LBL_LABELS_CANONICAL)
-        return fail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_CANONICAL is not available in this build");
     return LBL_OK;
 }
 
@@ -127,14 +125,16 @@ lbl_status check_size3d(lbl_ctx *ctx, uint32_t d, uint32_t h, uint32_t w)
     return LBL_OK;
 }
 
-size_t workspace2d(lbl_alg alg, uint32_t h, uint32_t w)
+size_t workspace2d(lbl_alg alg, uint32_t h, uint32_t w, lbl_label_mode mode)
 {
-    return alg == LBL_BLEQ ? bleq2d_workspace(h, w) : lbeq2d_workspace(h, w);
+    const bool canon = mode == LBL_LABELS_CANONICAL;
+    return alg == LBL_BLEQ ? bleq2d_workspace(h, w, canon) : lbeq2d_workspace(h, w, canon);
 }
 
-size_t workspace3d(lbl_alg alg, uint32_t d, uint32_t h, uint32_t w)
+size_t workspace3d(lbl_alg alg, uint32_t d, uint32_t h, uint32_t w, lbl_label_mode mode)
 {
-    return alg == LBL_BLEQ ? bleq3d_workspace(d, h, w) : lbeq3d_workspace(d, h, w);
+    const bool canon = mode == LBL_LABELS_CANONICAL;
+    return alg == LBL_BLEQ ? bleq3d_workspace(d, h, w, canon) : lbeq3d_workspace(d, h, w, canon);
 }
 
 hipStream_t pick_stream(lbl_ctx *ctx, void *stream)
@@ -295,7 +295,7 @@ extern "C" int lbl_last_kernel_times(lbl_ctx *ctx, float *us, const char **names
 static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s)
 {
     bool fresh = false;
-    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w), &fresh);
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w, ctx->opt.mode), &fresh);
     if (st != LBL_OK) return st;
     // every workspace starts with the count slots, which runs leave at zero
     if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
@@ -314,7 +314,7 @@ static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, 
 static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_t s)
 {
     bool fresh = false;
-    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace3d(alg, im.d, im.h, im.w), &fresh);
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace3d(alg, im.d, im.h, im.w, ctx->opt.mode), &fresh);
     if (st != LBL_OK) return st;
     // every workspace starts with the count slots, which runs leave at zero
     if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
@@ -445,7 +445,7 @@ extern "C" lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
         const lbl_image2d &im = images[i];
         if (!im.px || !im.labels) return fail(ctx, LBL_EINVAL, "lbl_label2d_each: null buffer");
         if ((st = check_size2d(ctx, alg, im.h, im.w, im.pitch_bytes)) != LBL_OK) return st;
-        need = std::max(need, workspace2d(alg, im.h, im.w));
+        need = std::max(need, workspace2d(alg, im.h, im.w, ctx->opt.mode));
     }
     DeviceGuard g(ctx->device);
     hipStream_t s = pick_stream(ctx, stream);

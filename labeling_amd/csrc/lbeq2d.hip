@@ -25,6 +25,7 @@
 // reference's 3D variant, LabelingAlgs.cl:641; the 2D reference is 0-based and
 // loses pixel (0,0), LabelingAlgs.cl:37,48-50).
 #include "lbl_internal.hpp"
+#include "canon.hpp"
 #include "uf_device.hpp"
 
 namespace lbl {
@@ -196,11 +197,15 @@ __global__ __launch_bounds__(256) void lbeq2d_flatten(uint32_t *L, uint32_t h, u
 
 // ---- 4. final analyze pass -------------------------------------------------------
 
-template <bool VEC>
-__global__ __launch_bounds__(256) void lbeq_final(uint32_t *__restrict__ L, uint64_t n, uint32_t *slots)
+// MARK: also write the canonical key bits, one bit per pixel set at roots (a
+// root is its component's minimum pixel index, i.e. its first pixel); the 16
+// lanes holding the 64 labels of one bits word assemble it.
+template <bool VEC, bool MARK>
+__global__ __launch_bounds__(256) void lbeq_final(uint32_t *__restrict__ L, uint64_t n, uint32_t *slots,
+                                                  unsigned long long *bits)
 {
     const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-    uint32_t roots = 0;
+    uint32_t roots = 0, rmask = 0;
     if (i0 < n) {
         uint32_t v[4];
         const bool full = VEC && i0 + 4 <= n;
@@ -216,6 +221,7 @@ __global__ __launch_bounds__(256) void lbeq_final(uint32_t *__restrict__ L, uint
             if (!v[k]) continue;
             if ((uint64_t)v[k] == i0 + k + 1) {
                 ++roots;
+                rmask |= 1u << k;
                 continue;
             }
             uint32_t r = v[k], vr = L[r - 1u];
@@ -235,29 +241,86 @@ __global__ __launch_bounds__(256) void lbeq_final(uint32_t *__restrict__ L, uint
                     if (i0 + k < n) L[i0 + k] = v[k];
         }
     }
+    if (MARK) {
+        const uint32_t sh = (uint32_t)(i0 & 63u);   // lanes 16k .. 16k+15 share a word
+        uint32_t lo = sh < 32 ? rmask << sh : 0u, hi = sh >= 32 ? rmask << (sh - 32) : 0u;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            lo |= __shfl_xor(lo, off, 64);
+            hi |= __shfl_xor(hi, off, 64);
+        }
+        if (sh == 0 && i0 < n) bits[i0 >> 6] = ((unsigned long long)hi << 32) | lo;
+    }
     if (slots) count_add(slots, roots);
+}
+
+// Canonical values in place of root labels (every label is its root's).
+template <bool VEC>
+__global__ __launch_bounds__(256) void lbeq_relabel(uint32_t *__restrict__ L, uint64_t n, Canon c)
+{
+    const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 >= n) return;
+    uint32_t v[4];
+    const bool full = VEC && i0 + 4 <= n;
+    if (full) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(L + i0);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+        for (uint32_t k = 0; k < 4; ++k) v[k] = i0 + k < n ? L[i0 + k] : 0u;
+    }
+    uint32_t prev = 0, prev_lab = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (!v[k]) continue;
+        if (v[k] != prev) {
+            prev = v[k];
+            prev_lab = canon_label(c, v[k] - 1u);
+        }
+        v[k] = prev_lab;
+    }
+    if (full) *reinterpret_cast<uint4 *>(L + i0) = make_uint4(v[0], v[1], v[2], v[3]);
+    else
+        for (uint32_t k = 0; k < 4; ++k)
+            if (i0 + k < n) L[i0 + k] = v[k];
 }
 
 }  // namespace
 
-size_t lbeq2d_workspace(uint32_t, uint32_t) { return kCountBytes; }
+size_t lbeq2d_workspace(uint32_t h, uint32_t w, bool canonical)
+{
+    return align256(kCountBytes) + (canonical ? canon_workspace((uint64_t)h * w, 0) : 0);
+}
 
 // Shared with the 3D label-equivalence path: the analyze pass over n labels,
-// then the component count (slots -> *count) when count is not null.
-hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, uint32_t *slots, hipStream_t s,
-                             KernelTimer *kt)
+// then the component count (slots -> *count) when count is not null, then,
+// with `canon_ws` (canonical numbering state after the count slots), the
+// canonical values.
+hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, uint32_t *slots, void *canon_ws,
+                             hipStream_t s, KernelTimer *kt)
 {
     const uint64_t threads = (n + 3) / 4;
     const dim3 grid((uint32_t)((threads + 255) / 256));
     uint32_t *sl = count ? slots : nullptr;
-    if (reinterpret_cast<uintptr_t>(labels) % 16 == 0)
-        hipLaunchKernelGGL(lbeq_final<true>, grid, dim3(256), 0, s, labels, n, sl);
-    else
-        hipLaunchKernelGGL(lbeq_final<false>, grid, dim3(256), 0, s, labels, n, sl);
+    const bool vec = reinterpret_cast<uintptr_t>(labels) % 16 == 0;
+    const Canon c = canon_ws ? canon_layout(canon_ws, n, 0) : Canon{};
+    if (canon_ws) {
+        if (vec) hipLaunchKernelGGL((lbeq_final<true, true>), grid, dim3(256), 0, s, labels, n, sl, c.bits);
+        else hipLaunchKernelGGL((lbeq_final<false, true>), grid, dim3(256), 0, s, labels, n, sl, c.bits);
+    } else {
+        if (vec) hipLaunchKernelGGL((lbeq_final<true, false>), grid, dim3(256), 0, s, labels, n, sl, nullptr);
+        else hipLaunchKernelGGL((lbeq_final<false, false>), grid, dim3(256), 0, s, labels, n, sl, nullptr);
+    }
     if (kt) kt->mark(s, "lbeq_final");
     if (count) {
         hipLaunchKernelGGL(count_reduce_kernel<0>, dim3(1), dim3(kCountSlots), 0, s, slots, count);
         if (kt) kt->mark(s, "count_reduce");
+    }
+    if (canon_ws) {
+        const hipError_t e = canon_scan_launch(c, s, kt);
+        if (e != hipSuccess) return e;
+        if (vec) hipLaunchKernelGGL(lbeq_relabel<true>, grid, dim3(256), 0, s, labels, n, c);
+        else hipLaunchKernelGGL(lbeq_relabel<false>, grid, dim3(256), 0, s, labels, n, c);
+        if (kt) kt->mark(s, "lbeq_relabel");
     }
     return hipGetLastError();
 }
@@ -283,7 +346,8 @@ hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws,
     if (kt) kt->mark(s, "lbeq2d_merge");
     hipLaunchKernelGGL(lbeq2d_flatten, dim3(border_grid), dim3(256), 0, s, im.labels, im.h, im.w, tiles_x, ntiles);
     if (kt) kt->mark(s, "lbeq2d_flatten");
-    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.h * im.w, im.count, slots, s, kt);
+    void *canon_ws = opt.mode == LBL_LABELS_CANONICAL ? static_cast<uint8_t *>(ws) + align256(kCountBytes) : nullptr;
+    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.h * im.w, im.count, slots, canon_ws, s, kt);
     return e != hipSuccess ? e : hipGetLastError();
 }
 

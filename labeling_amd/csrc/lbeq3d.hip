@@ -15,12 +15,13 @@
 //   3. flatten one thread per tile-face voxel: tile-local root -> global root.
 //   4. final   the analyze pass over the volume (shared with 2D).
 #include "lbl_internal.hpp"
+#include "canon.hpp"
 #include "uf_device.hpp"
 
 namespace lbl {
 
-hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, uint32_t *slots, hipStream_t s,
-                             KernelTimer *kt);
+hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, uint32_t *slots, void *canon_ws,
+                             hipStream_t s, KernelTimer *kt);
 
 namespace {
 
@@ -193,7 +194,10 @@ __global__ __launch_bounds__(256) void lbeq3d_flatten(uint32_t *L, Grid3 g)
 
 }  // namespace
 
-size_t lbeq3d_workspace(uint32_t, uint32_t, uint32_t) { return kCountBytes; }
+size_t lbeq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical)
+{
+    return align256(kCountBytes) + (canonical ? canon_workspace((uint64_t)d * h * w, 0) : 0);
+}
 
 hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
 {
@@ -214,7 +218,8 @@ hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream
     if (kt) kt->mark(s, "lbeq3d_merge");
     hipLaunchKernelGGL(lbeq3d_flatten, dim3(face_grid), dim3(256), 0, s, im.labels, g);
     if (kt) kt->mark(s, "lbeq3d_flatten");
-    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.d * im.h * im.w, im.count, slots, s, kt);
+    void *canon_ws = opt.mode == LBL_LABELS_CANONICAL ? static_cast<uint8_t *>(ws) + align256(kCountBytes) : nullptr;
+    hipError_t e = lbeq_final_launch(im.labels, (uint64_t)im.d * im.h * im.w, im.count, slots, canon_ws, s, kt);
     return e != hipSuccess ? e : hipGetLastError();
 }
 

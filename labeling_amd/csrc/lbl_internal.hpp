@@ -66,10 +66,11 @@ struct KernelTimer {
 };
 
 // Workspace sizes (bytes) needed by each path; 256-byte aligned sub-buffers.
-size_t bleq2d_workspace(uint32_t h, uint32_t w);
-size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w);
-size_t lbeq2d_workspace(uint32_t h, uint32_t w);
-size_t lbeq3d_workspace(uint32_t d, uint32_t h, uint32_t w);
+// `canonical`: room for canonical numbering (LBL_LABELS_CANONICAL) as well.
+size_t bleq2d_workspace(uint32_t h, uint32_t w, bool canonical = false);
+size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical = false);
+size_t lbeq2d_workspace(uint32_t h, uint32_t w, bool canonical = false);
+size_t lbeq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical = false);
 
 // Enqueue the full labeling pipeline of one image on `s`.  `ws` must hold the
 // workspace size above.  Return hipSuccess or the first launch error.
