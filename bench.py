@@ -136,8 +136,21 @@ def bench_2d(L, torch, lab, a):
             ms2, _ = timed_steps(torch, stream, lambda i: step(i, "lbeq", conn), len(inputs), 2, 12)
             lbeq[f"conn{conn}_gpx_s"] = round(px * 12 / (ms2 / 1e3) / 1e9, 2)
         out["lbeq2d"] = lbeq
+        out["canonical"] = canonical_line(L, torch, lab, stream, step, names, px, a)
     del d_in
     return out
+
+
+def canonical_line(L, torch, lab, stream, step, names, units, a, steps=30):
+    """The same pipeline in LBL_LABELS_CANONICAL mode (device canonical numbering)."""
+    lab.set_mode(L.LABELS_CANONICAL)
+    try:
+        ms, _ = timed_steps(torch, stream, step, len(names), 3, steps)
+        kavg, _ = kernel_breakdown(lab, step, names, len(names) * 2)
+    finally:
+        lab.set_mode(L.LABELS_ROOT)
+    return {"g_units_s": round(units * steps / (ms / 1e3) / 1e9, 2), "ms_per_step": round(ms / steps, 4),
+            "path_us": round(sum(kavg.values()), 2), "kernels_us": {k: round(v, 2) for k, v in kavg.items()}}
 
 
 # ---- 3D ---------------------------------------------------------------------------
@@ -167,6 +180,8 @@ def bench_3d(L, torch, lab, a):
         r["roofline"]["path_us"] = round(path_us, 2)
         r["roofline"]["kernels_us"] = {k: round(v, 2) for k, v in kavg.items()}
         r["per_input_us"] = {k: round(v, 2) for k, v in per_input.items()}
+        if alg == "bleq":
+            r["canonical"] = canonical_line(L, torch, lab, stream, step, [n for n, _ in vols], s ** 3, a, steps=15)
         res[alg] = r
     del d_in
     return res
@@ -392,6 +407,8 @@ def main():
     if not a.quick:
         if "lbeq2d" in r2:
             extra["lbeq2d_8192"] = r2.pop("lbeq2d")
+        if "canonical" in r2:
+            extra["canonical_2d_8192"] = r2.pop("canonical")
         if "3d" not in skip:
             extra["volume_512"] = bench_3d(L, torch, lab, a)
         if "batch" not in skip:

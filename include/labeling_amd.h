@@ -93,8 +93,9 @@ const char *lbl_status_string(lbl_status s);
 
 lbl_status lbl_set_label_mode(lbl_ctx *ctx, lbl_label_mode mode);
 
-/* Debug knob: cap on the number of tile-local label-equivalence sweeps (lbeq)
- * per tile; exceeding it returns LBL_EITER.  0 = no cap (default). */
+/* Debug knob: cap on the number of tile-local label-equivalence sweeps per
+ * tile, for the one path that sweeps (lbeq 2D, 4-connectivity); exceeding it
+ * returns LBL_EITER.  0 = no cap (default). */
 lbl_status lbl_set_iteration_cap(lbl_ctx *ctx, uint32_t cap);
 
 /* Registry lookup by the reference's algorithm names, mirroring

@@ -245,7 +245,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
                 cur = (v[kk] & kResolved) ? v[kk] : g0 + kk;
                 const bool root = (cur & ~kResolved) == g0 + kk;
                 roots += root;
-                if (root && (G.flags & kCanon)) G.canon.key[g0 + kk] = ~0u;   // global roots are tile-local roots
+                if (root && (G.flags & kKeys)) G.canon.key[g0 + kk] = ~0u;   // global roots are tile-local roots
             }
             const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
             out[kk] = m ? (cur << 4) | m : (g0 + kk) << 4;   // << 4 drops the tag bit
@@ -556,7 +556,12 @@ __device__ __forceinline__ void lane_blocks(const Geom2D &G, uint32_t by, uint32
     }
 }
 
-template <class Src>
+// Label values the final kernel writes: fixed per launch (MODE), or per image
+// from its flags (kModeAny, batches).
+// kModeFirst: 1 + the component's first pixel index (the label equivalence values).
+enum : uint32_t { kModeRoot = 0, kModeRef = 1, kModeCanon = 2, kModeFirst = 3, kModeAny = 4 };
+
+template <class Src, uint32_t MODE>
 __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
 {
     uint32_t lw;
@@ -569,7 +574,8 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
 
     if (by < G.bh && bxs < bw) {
         uint32_t lab[FINAL_BLOCKS];
-        const bool ref = (G.flags & kRefLabels) != 0, canon = (G.flags & kCanon) != 0;
+        const bool ref = MODE == kModeRef || (MODE == kModeAny && (G.flags & kRefLabels));
+        const bool canon = MODE == kModeCanon;
         uint32_t prev_root = ~0u, prev_lab = 0;
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
@@ -580,7 +586,8 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
                 prev_root = r;
                 // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
                 prev_lab = canon ? canon_label(G.canon, G.canon.key[r])
-                                 : (ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u);
+                                 : MODE == kModeFirst ? G.canon.key[r] + 1u
+                                                      : (ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u);
             }
             lab[k] = prev_lab;
         }
@@ -604,15 +611,17 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
     if (lw == 0 && G.count && threadIdx.x < kCountSlots) count_reduce(G.slots, G.count);
 }
 
-// ---- canonical numbering (kCanon): keys, then bits at the keys -----------------------
+// ---- first-pixel keys; canonical bits -------------------------------------------------
 //
 // A component's first pixel lies in its topmost block row, which is the block
 // row of its root (the minimum block index); each block of that row in the
-// component offers its own first pixel, and the smallest offer is the
-// component's key: first-pixel row * bw + block column.  The keys kernel also
-// clears the key bits; the mark kernel then sets the bit of every root's key.
+// component offers its own first pixel (its lowest mask bit), and the smallest
+// offer is the component's key: the pixel index (PIXEL_KEY; 1 + that is the
+// label equivalence value), or first-pixel row * bw + block column, the bit
+// position of canonical numbering.  The keys kernel also clears the key bits
+// (canon.nwords words); the mark kernel then sets the bit of every root's key.
 
-template <class Src>
+template <class Src, bool PIXEL_KEY>
 __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
 {
     uint32_t lw;
@@ -635,7 +644,8 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
     for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
         const uint32_t m = v[k] & 15u;
         if (!m || root[k] - row0 >= bw) continue;   // not in its component's top block row
-        const uint32_t key = (2 * by + ((m & 3u) ? 0u : 1u)) * bw + bxs + k;
+        const uint32_t bit = __builtin_ctz(m), y = 2 * by + (bit >> 1);
+        const uint32_t key = PIXEL_KEY ? y * G.w + 2 * (bxs + k) + (bit & 1u) : y * bw + bxs + k;
         if (r0 == ~0u || r0 == root[k]) {
             r0 = root[k];
             k0 = min(k0, key);
@@ -688,7 +698,7 @@ struct Variant {
 
     template <class Src>
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
-                             KernelTimer *kt, bool flatten, const Canon *canon)
+                             KernelTimer *kt, bool flatten, const Canon *canon, uint32_t mode)
     {
         const uint32_t merge_grid =
             (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + MERGE_THREADS - 1) / MERGE_THREADS);
@@ -701,33 +711,44 @@ struct Variant {
             hipLaunchKernelGGL((bleq2d_flatten<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
             if (kt) kt->mark(s, "bleq2d_flatten");
         }
+        if (mode == kModeFirst) {
+            hipLaunchKernelGGL((bleq2d_keys<Src, true>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+            if (kt) kt->mark(s, "bleq2d_keys");
+        }
         if (canon) {
-            hipLaunchKernelGGL((bleq2d_keys<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+            hipLaunchKernelGGL((bleq2d_keys<Src, false>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_keys");
             hipLaunchKernelGGL((bleq2d_mark<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_mark");
             const hipError_t e = canon_scan_launch(*canon, s, kt);
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL((bleq2d_final<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+        switch (mode) {
+        case kModeRoot: hipLaunchKernelGGL((bleq2d_final<Src, kModeRoot>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        case kModeRef: hipLaunchKernelGGL((bleq2d_final<Src, kModeRef>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        case kModeCanon: hipLaunchKernelGGL((bleq2d_final<Src, kModeCanon>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        case kModeFirst: hipLaunchKernelGGL((bleq2d_final<Src, kModeFirst>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        default: hipLaunchKernelGGL((bleq2d_final<Src, kModeAny>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        }
         if (kt) kt->mark(s, "bleq2d_final");
         return hipGetLastError();
     }
 };
 
-// canon: the single image's canonical state when it asks for canonical labels
+// canon: the single image's canonical state when it asks for canonical labels;
+// mode: the label values (kMode*)
 template <class Src>
 hipError_t dispatch(uint32_t tile_id, const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
-                    KernelTimer *kt, bool flatten, const Canon *canon = nullptr)
+                    KernelTimer *kt, bool flatten, const Canon *canon, uint32_t mode)
 {
     switch (tile_id) {
-    case 1: return Variant<64, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
-    case 2: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
-    case 3: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
-    case 4: return Variant<256, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
-    case 5: return Variant<512, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
-    case 6: return Variant<128, 128>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
-    default: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon);
+    case 1: return Variant<64, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
+    case 2: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
+    case 3: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
+    case 4: return Variant<256, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
+    case 5: return Variant<512, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
+    case 6: return Variant<128, 128>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
+    default: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
     }
 }
 
@@ -746,7 +767,7 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
     g.xgroups = (g.bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
     g.nfinal = g.xgroups * g.bh;
     g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
-    g.flags &= kRefLabels | kCanon;
+    g.flags &= kRefLabels | kKeys;
     if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
     if (g.w % 8 == 0 && reinterpret_cast<uintptr_t>(g.labels) % 16 == 0) g.flags |= kVecOut;
 }
@@ -758,7 +779,7 @@ size_t bleq2d_workspace(uint32_t h, uint32_t w, bool canonical)
            (canonical ? canon_workspace((uint64_t)h * bw, bw * bh) : 0);
 }
 
-hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
+hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt, bool first)
 {
     One one;
     Geom2D &g = one.g;
@@ -771,13 +792,19 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
     g.count = im.count;
     g.slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
     g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
-    g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : (opt.mode == LBL_LABELS_CANONICAL ? kCanon : 0u);
+    g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
     bleq2d_geometry(g, opt.tile2d);
-    if (g.flags & kCanon)
+    const bool canon = opt.mode == LBL_LABELS_CANONICAL;
+    if (canon || first) {
         g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256((size_t)g.bw * g.bh * sizeof(uint32_t)),
                                (uint64_t)g.h * g.bw, (uint64_t)g.bw * g.bh);
+        if (!canon) g.canon.nwords = 0;   // keys only
+        g.flags |= kKeys;   // the tile kernel initialises the keys of roots
+    }
     if (kt) kt->begin(s);
-    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, (g.flags & kCanon) ? &g.canon : nullptr);
+    // canonical numbering uses bit-position keys for both label sets (the same components)
+    const uint32_t mode = canon ? kModeCanon : (first ? kModeFirst : ((g.flags & kRefLabels) ? kModeRef : kModeRoot));
+    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, canon ? &g.canon : nullptr, mode);
 }
 
 hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,
@@ -785,7 +812,7 @@ hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_ti
 {
     Many many{d_geoms, n};
     if (kt) kt->begin(s);
-    return dispatch(tile_id, many, total_tiles, total_final, s, kt, flatten);
+    return dispatch(tile_id, many, total_tiles, total_final, s, kt, flatten, nullptr, kModeAny);
 }
 
 }  // namespace lbl

@@ -10,7 +10,7 @@ enum : uint32_t {
     kVecIn = 1u,       // image rows start 16-byte aligned (pitch and base)
     kVecOut = 2u,      // label rows start 16-byte aligned (w % 8 == 0, base)
     kRefLabels = 4u,   // write the reference's raw label values
-    kCanon = 8u,       // write canonical labels (canon.hpp)
+    kKeys = 8u,        // keep first-pixel keys of roots (canonical or label equivalence values)
 };
 
 // One image as the kernels see it (also the element of a batch's device table).

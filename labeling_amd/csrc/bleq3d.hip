@@ -472,12 +472,19 @@ __device__ __forceinline__ void lane_blocks3(const uint32_t *__restrict__ P, con
     }
 }
 
-template <bool VEC>
+// Label values the final kernel writes.
+enum : uint32_t {
+    kModeRoot = 0,    // 1 + root block index
+    kModeCanon = 1,   // canonical numbering (keys: bit positions)
+    kModeFirst = 2,   // 1 + the component's first voxel index (keys: voxel indices): the lbeq values
+};
+
+template <bool VEC, uint32_t MODE>
 __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
                                                           uint32_t *__restrict__ labels, uint32_t d, uint32_t h,
                                                           uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
                                                           uint32_t xgroups, uint32_t *count, uint32_t *slots,
-                                                          Canon canon, bool canonical)
+                                                          Canon canon)
 {
     const uint32_t row = blockIdx.x / xgroups;           // block row index bz * bh + by
     const uint32_t by = row % bh, bz = row / bh;
@@ -496,7 +503,8 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
             if (!mv[k]) continue;
             if (root[k] != prev_root) {
                 prev_root = root[k];
-                prev_lab = canonical ? canon_label(canon, canon.key[root[k]]) : root[k] + 1u;
+                prev_lab = MODE == kModeCanon ? canon_label(canon, canon.key[root[k]])
+                                              : (MODE == kModeFirst ? canon.key[root[k]] + 1u : root[k] + 1u);
             }
             lab[k] = prev_lab;
         }
@@ -525,18 +533,20 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
     if (blockIdx.x == 0 && count && threadIdx.x < kCountSlots) count_reduce(slots, count);
 }
 
-// ---- canonical numbering: keys, then bits at the keys -----------------------------
+// ---- first-voxel keys; canonical bits -----------------------------------------------
 //
 // A component's first voxel ([z][y][x] raster order) lies in its front block
 // plane, the plane of its root (the minimum block index); each block of that
 // plane in the component offers its own first voxel (the lowest mask bit, as
 // bit dx + 2 dy + 4 dz orders the block's voxels), and the smallest offer is
-// the component's key: (z * h + y) * bw + block column.  The keys kernel also
-// clears the key bits; the mark kernel then sets the bit of every root's key.
+// the component's key: its voxel index (VOXEL_KEY; 1 + that is the label
+// equivalence value), or (z * h + y) * bw + block column, the bit position of
+// canonical numbering.  The keys kernel also clears the key bits (c.nwords
+// words); the mark kernel then sets the bit of every root's key.
 
-template <bool VEC>
+template <bool VEC, bool VOXEL_KEY>
 __global__ __launch_bounds__(NT_FINAL) void bleq3d_keys(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
-                                                         uint32_t h, uint32_t bd, uint32_t bh, uint32_t bw,
+                                                         uint32_t h, uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
                                                          uint32_t xgroups, Canon c)
 {
     const uint64_t t = (uint64_t)blockIdx.x * NT_FINAL + threadIdx.x;   // covers every word
@@ -555,8 +565,9 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_keys(const uint32_t *__restri
     for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
         const uint32_t m = mv[k];
         if (!m || root[k] - plane0 >= plane) continue;   // not in its component's front plane
-        const uint32_t bit = __builtin_ctz(m), dy = (bit >> 1) & 1u, dz = bit >> 2;
-        const uint32_t key = ((2 * bz + dz) * h + 2 * by + dy) * bw + bxs + k;
+        const uint32_t bit = __builtin_ctz(m), dx = bit & 1u, dy = (bit >> 1) & 1u, dz = bit >> 2;
+        const uint32_t zy = (2 * bz + dz) * h + 2 * by + dy;
+        const uint32_t key = VOXEL_KEY ? zy * w + 2 * (bxs + k) + dx : zy * bw + bxs + k;
         if (r0 == ~0u || r0 == root[k]) {
             r0 = root[k];
             k0 = min(k0, key);
@@ -599,7 +610,13 @@ size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical)
            (canonical ? canon_workspace((uint64_t)d * h * ((w + 1ull) / 2), nb) : 0);
 }
 
-hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
+// The block pipeline for both 3D paths: bleq values (1 + root block), or with
+// `first` the label-equivalence values (1 + first voxel index: for
+// 26-connectivity the voxels of a 2x2x2 block are always connected, so the
+// block forest has the voxel forest's components); canonical numbering for
+// either in LBL_LABELS_CANONICAL mode.
+static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt,
+                               bool first)
 {
     const uint32_t bd = (im.d + 1u) / 2, bh = (im.h + 1u) / 2, bw = (im.w + 1u) / 2;
     const uint64_t nb = (uint64_t)bd * bh * bw;
@@ -609,8 +626,9 @@ hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream
     uint32_t *slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
     uint32_t *P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
     uint8_t *M = reinterpret_cast<uint8_t *>(P) + align256(nb * sizeof(uint32_t));
-    const bool canonical = opt.mode == LBL_LABELS_CANONICAL;
-    const Canon canon = canonical ? canon_layout(M + align256(nb), (uint64_t)im.d * im.h * bw, nb) : Canon{};
+    const bool canonical = opt.mode == LBL_LABELS_CANONICAL, keys = canonical || first;
+    Canon canon = keys ? canon_layout(M + align256(nb), (uint64_t)im.d * im.h * bw, nb) : Canon{};
+    if (!canonical) canon.nwords = 0;   // keys only
     const bool vec_in = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
     const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
     const uint32_t merge_grid = (uint32_t)(((uint64_t)g.ntiles * (BAR_LANES + X_LANES) + MERGE_THREADS - 1) / MERGE_THREADS);
@@ -628,23 +646,46 @@ hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream
     hipLaunchKernelGGL(bleq3d_merge, dim3(merge_grid), dim3(MERGE_THREADS), 0, s, P, M, g, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_merge");
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
-    if (canonical) {
-        if (vec_out) hipLaunchKernelGGL(bleq3d_keys<true>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.h, bd, bh, bw, xgroups, canon);
-        else hipLaunchKernelGGL(bleq3d_keys<false>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.h, bd, bh, bw, xgroups, canon);
+    if (keys) {
+        // canonical numbering uses bit-position keys (the same for both paths); otherwise voxel indices
+        auto kern = canonical ? (vec_out ? bleq3d_keys<true, false> : bleq3d_keys<false, false>)
+                              : (vec_out ? bleq3d_keys<true, true> : bleq3d_keys<false, true>);
+        hipLaunchKernelGGL(kern, fgrid, dim3(NT_FINAL), 0, s, P, M, im.h, im.w, bd, bh, bw, xgroups, canon);
         if (kt) kt->mark(s, "bleq3d_keys");
+    }
+    if (canonical) {
         hipLaunchKernelGGL(bleq3d_mark, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, P, M, nb, canon);
         if (kt) kt->mark(s, "bleq3d_mark");
         const hipError_t e = canon_scan_launch(canon, s, kt);
         if (e != hipSuccess) return e;
     }
-    if (vec_out)
-        hipLaunchKernelGGL(bleq3d_final<true>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd, bh,
-                           bw, xgroups, im.count, slots, canon, canonical);
-    else
-        hipLaunchKernelGGL(bleq3d_final<false>, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd,
-                           bh, bw, xgroups, im.count, slots, canon, canonical);
+    const uint32_t mode = canonical ? kModeCanon : (first ? kModeFirst : kModeRoot);
+    auto fin = vec_out ? (mode == kModeCanon ? bleq3d_final<true, kModeCanon>
+                                             : (mode == kModeFirst ? bleq3d_final<true, kModeFirst> : bleq3d_final<true, kModeRoot>))
+                       : (mode == kModeCanon ? bleq3d_final<false, kModeCanon>
+                                             : (mode == kModeFirst ? bleq3d_final<false, kModeFirst> : bleq3d_final<false, kModeRoot>));
+    hipLaunchKernelGGL(fin, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd, bh, bw, xgroups,
+                       im.count, slots, canon);
     if (kt) kt->mark(s, "bleq3d_final");
     return hipGetLastError();
+}
+
+hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
+{
+    return run_blocks3d(im, opt, ws, s, kt, false);
+}
+
+// Label equivalence in 3D (TOCLLabelEquivalence3D, LabelingAlgs.cpp:1362-1396;
+// LBEQ3D_*, LabelingAlgs.cl:635-734) has the same components and labels them
+// 1 + minimum voxel index, which is the first voxel in raster order.
+hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
+{
+    return run_blocks3d(im, opt, ws, s, kt, true);
+}
+
+size_t lbeq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool)
+{
+    return bleq3d_workspace(d, h, w, true);
 }
 
 }  // namespace lbl

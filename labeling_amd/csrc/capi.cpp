@@ -301,7 +301,9 @@ static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, 
     if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
     KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
     Options opt = ctx->opt;
-    auto it = ctx->tuning.find(TuneKey{(int)alg, 0, size_class_of((uint64_t)im.h * im.w)});
+    // lbeq with 8-connectivity runs the bleq block pipeline: bleq's tuning applies
+    const int talg = (alg == LBL_LBEQ && conn == 8) ? (int)LBL_BLEQ : (int)alg;
+    auto it = ctx->tuning.find(TuneKey{talg, 0, size_class_of((uint64_t)im.h * im.w)});
     if (it != ctx->tuning.end()) {   // autotuned launch configuration for this size class
         opt.tile2d = it->second.tile;
         opt.flatten = it->second.flatten;

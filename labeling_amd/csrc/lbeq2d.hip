@@ -24,6 +24,13 @@
 // Labels: 1 + raster index of the component's minimum pixel (1-based like the
 // reference's 3D variant, LabelingAlgs.cl:641; the 2D reference is 0-based and
 // loses pixel (0,0), LabelingAlgs.cl:37,48-50).
+//
+// These kernels run 4-connectivity (and 8-connectivity beyond the 2^28-block
+// limit of the block forest); 8-connectivity otherwise runs on the bleq block
+// pipeline with a first-pixel key pass (bleq2d.hip), which gives the same
+// labels.
+#include <algorithm>
+
 #include "lbl_internal.hpp"
 #include "canon.hpp"
 #include "uf_device.hpp"
@@ -288,7 +295,8 @@ __global__ __launch_bounds__(256) void lbeq_relabel(uint32_t *__restrict__ L, ui
 
 size_t lbeq2d_workspace(uint32_t h, uint32_t w, bool canonical)
 {
-    return align256(kCountBytes) + (canonical ? canon_workspace((uint64_t)h * w, 0) : 0);
+    const size_t own = align256(kCountBytes) + (canonical ? canon_workspace((uint64_t)h * w, 0) : 0);
+    return std::max(own, bleq2d_workspace(h, w, true));   // 8-connectivity runs on the block forest
 }
 
 // Shared with the 3D label-equivalence path: the analyze pass over n labels,
@@ -327,6 +335,11 @@ hipError_t lbeq_final_launch(uint32_t *labels, uint64_t n, uint32_t *count, uint
 
 hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
 {
+    // 8-connectivity: the pixels of a 2x2 block are always connected, so the
+    // bleq block forest has the same components; its first-pixel key pass gives
+    // the label equivalence values (1 + minimum pixel index)
+    if (conn == 8 && ((im.h + 1ull) / 2) * ((im.w + 1ull) / 2) <= kMaxBlocks2D)
+        return bleq2d_run(im, opt, ws, s, kt, true);
     const uint32_t tiles_x = (im.w + TW - 1) / TW, tiles_y = (im.h + TH - 1) / TH;
     const uint32_t ntiles = tiles_x * tiles_y;
     const bool vec = (im.pitch % 16 == 0) && (reinterpret_cast<uintptr_t>(im.px) % 16 == 0);

@@ -74,7 +74,10 @@ size_t lbeq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical = fal
 
 // Enqueue the full labeling pipeline of one image on `s`.  `ws` must hold the
 // workspace size above.  Return hipSuccess or the first launch error.
-hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+// first: label equivalence values (1 + first pixel index) from the bleq block
+// forest, for 8-connectivity (the pixels of a 2x2 block are 8-connected).
+hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt,
+                      bool first = false);
 hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
