@@ -109,6 +109,16 @@ struct Many {
     }
 };
 
+// First-pixel key of block (bx, by) with mask m != 0 (bit 1 << (dx + 2 dy)):
+// its first pixel's index (kPixelKeys), or first-pixel row * bw + bx, the bit
+// position of canonical numbering.  Both order blocks' first pixels as the
+// raster does, and no two blocks share a key.
+__device__ __forceinline__ uint32_t block_key(const Geom2D &G, uint32_t by, uint32_t bx, uint32_t m)
+{
+    const uint32_t bit = __builtin_ctz(m), y = 2 * by + (bit >> 1);
+    return (G.flags & kPixelKeys) ? y * G.w + 2 * bx + (bit & 1u) : y * G.bw + bx;
+}
+
 // ---- 1. tile kernel -------------------------------------------------------------
 //
 // A tile is TBW x TBH blocks; one thread ("lane") per 32-pixel-wide segment of
@@ -241,13 +251,15 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
         uint32_t out[LANE_BLOCKS], cur = 0;
 #pragma unroll
         for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {
+            const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
             if (st & (3u << (2 * kk))) {
                 cur = (v[kk] & kResolved) ? v[kk] : g0 + kk;
                 const bool root = (cur & ~kResolved) == g0 + kk;
                 roots += root;
-                if (root && (G.flags & kKeys)) G.canon.key[g0 + kk] = ~0u;   // global roots are tile-local roots
+                // first-pixel keys start at the root block's own first pixel
+                // (global roots are tile-local roots)
+                if (root && (G.flags & kKeys)) G.canon.key[g0 + kk] = block_key(G, by, bxs + kk, m);
             }
-            const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
             out[kk] = m ? (cur << 4) | m : (g0 + kk) << 4;   // << 4 drops the tag bit
         }
         uint32_t *dst = G.P + (size_t)by * bw + bxs;
@@ -621,7 +633,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
 // position of canonical numbering.  The keys kernel also clears the key bits
 // (canon.nwords words); the mark kernel then sets the bit of every root's key.
 
-template <class Src, bool PIXEL_KEY>
+template <class Src>
 __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
 {
     uint32_t lw;
@@ -633,24 +645,35 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
     const uint32_t bw = G.bw;
     const uint32_t by = lw / G.xgroups;
     const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
-    uint32_t v[FINAL_BLOCKS], root[FINAL_BLOCKS];
-    lane_blocks(G, by, bxs, v, root);
+    uint32_t v[FINAL_BLOCKS] = {};
+    if (by < G.bh && bxs < bw)
+        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = bxs + k < bw ? G.P[(size_t)by * bw + bxs + k] : 0u;
 
     // this lane's offers: (root, key) of its first candidate block, and of a
-    // second one with another root
+    // second one with another root.  Roots are never above parents (parent <=
+    // self), so only a block whose parent lies in this block row can have its
+    // root here: the others need no chase.  A root's key starts at its own
+    // block's key, so only offers below the current key are made.
     const uint32_t row0 = by * bw;
-    uint32_t r0 = ~0u, k0 = ~0u, r1 = ~0u, k1 = ~0u;
+    uint32_t r0 = ~0u, k0 = ~0u, r1 = ~0u, k1 = ~0u, prev_p = ~0u, prev_r = 0, prev_k = 0;
 #pragma unroll
     for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
-        const uint32_t m = v[k] & 15u;
-        if (!m || root[k] - row0 >= bw) continue;   // not in its component's top block row
-        const uint32_t bit = __builtin_ctz(m), y = 2 * by + (bit >> 1);
-        const uint32_t key = PIXEL_KEY ? y * G.w + 2 * (bxs + k) + (bit & 1u) : y * bw + bxs + k;
-        if (r0 == ~0u || r0 == root[k]) {
-            r0 = root[k];
+        const uint32_t m = v[k] & 15u, p = v[k] >> 4;
+        if (!m || p < row0) continue;
+        if (p != prev_p) {
+            prev_p = p;
+            prev_r = Forest::find_final(G.P, p);
+            prev_k = prev_r >= row0 ? c.key[prev_r] : 0u;
+        }
+        const uint32_t r = prev_r;
+        if (r < row0) continue;   // not in its component's top block row
+        const uint32_t key = block_key(G, by, bxs + k, m);
+        if (key >= prev_k) continue;
+        if (r0 == ~0u || r0 == r) {
+            r0 = r;
             k0 = min(k0, key);
         } else {
-            r1 = root[k];
+            r1 = r;
             k1 = key;
         }
     }
@@ -711,13 +734,11 @@ struct Variant {
             hipLaunchKernelGGL((bleq2d_flatten<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
             if (kt) kt->mark(s, "bleq2d_flatten");
         }
-        if (mode == kModeFirst) {
-            hipLaunchKernelGGL((bleq2d_keys<Src, true>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+        if (mode == kModeFirst || canon) {
+            hipLaunchKernelGGL((bleq2d_keys<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_keys");
         }
         if (canon) {
-            hipLaunchKernelGGL((bleq2d_keys<Src, false>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
-            if (kt) kt->mark(s, "bleq2d_keys");
             hipLaunchKernelGGL((bleq2d_mark<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_mark");
             const hipError_t e = canon_scan_launch(*canon, s, kt);
@@ -767,7 +788,7 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
     g.xgroups = (g.bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
     g.nfinal = g.xgroups * g.bh;
     g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
-    g.flags &= kRefLabels | kKeys;
+    g.flags &= kRefLabels | kKeys | kPixelKeys;
     if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
     if (g.w % 8 == 0 && reinterpret_cast<uintptr_t>(g.labels) % 16 == 0) g.flags |= kVecOut;
 }
@@ -799,10 +820,11 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
         g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256((size_t)g.bw * g.bh * sizeof(uint32_t)),
                                (uint64_t)g.h * g.bw, (uint64_t)g.bw * g.bh);
         if (!canon) g.canon.nwords = 0;   // keys only
-        g.flags |= kKeys;   // the tile kernel initialises the keys of roots
+        // the tile kernel initialises the keys of roots; canonical numbering uses
+        // bit-position keys for both label sets (the same components)
+        g.flags |= canon ? kKeys : (kKeys | kPixelKeys);
     }
     if (kt) kt->begin(s);
-    // canonical numbering uses bit-position keys for both label sets (the same components)
     const uint32_t mode = canon ? kModeCanon : (first ? kModeFirst : ((g.flags & kRefLabels) ? kModeRef : kModeRoot));
     return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, canon ? &g.canon : nullptr, mode);
 }

@@ -11,6 +11,7 @@ enum : uint32_t {
     kVecOut = 2u,      // label rows start 16-byte aligned (w % 8 == 0, base)
     kRefLabels = 4u,   // write the reference's raw label values
     kKeys = 8u,        // keep first-pixel keys of roots (canonical or label equivalence values)
+    kPixelKeys = 16u,  // keys are pixel indices (else canonical bit positions)
 };
 
 // One image as the kernels see it (also the element of a batch's device table).

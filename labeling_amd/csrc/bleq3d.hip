@@ -108,11 +108,23 @@ struct Tile3 {
     static __device__ __forceinline__ uint32_t at(uint32_t gz, uint32_t gy, uint32_t gx) { return (gz * GY + gy) * GX + gx; }
 };
 
+// First-voxel key of block (bx, by, bz) with mask m != 0 (bit 1 << (dx + 2 dy
+// + 4 dz), so the lowest bit is the first voxel): its voxel index (voxel_key),
+// or (z * h + y) * bw + bx, the bit position of canonical numbering.  Both
+// order blocks' first voxels as the raster does, and no two blocks share a key.
+__device__ __forceinline__ uint32_t block_key3(uint32_t h, uint32_t w, uint32_t bw, uint32_t bz, uint32_t by,
+                                               uint32_t bx, uint32_t m, bool voxel_key)
+{
+    const uint32_t bit = __builtin_ctz(m), dx = bit & 1u, dy = (bit >> 1) & 1u, dz = bit >> 2;
+    const uint32_t zy = (2 * bz + dz) * h + 2 * by + dy;
+    return voxel_key ? zy * w + 2 * bx + dx : zy * bw + bx;
+}
+
 template <uint32_t TBW, uint32_t TBY, uint32_t TBZ, bool VEC>
 __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     const uint8_t *__restrict__ vox, uint32_t d, uint32_t h, uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
     uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P, uint8_t *__restrict__ M, uint32_t *count,
-    uint32_t *slots, uint32_t *ckey)
+    uint32_t *slots, uint32_t *ckey, bool voxel_key)
 {
     using T = Tile3<TBW, TBY, TBZ>;
     constexpr uint32_t LPB = T::LPB;
@@ -246,13 +258,15 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         uint32_t out[kLaneBlocks], mk[kLaneBlocks / 4] = {0, 0, 0, 0}, cur = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kLaneBlocks; ++k) {
+            const uint32_t m = ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
+                               (((e >> (2 * k)) & 3u) << 6);
             if (st & (3u << (2 * k))) {
                 cur = (v[k] & kResolved) ? (v[k] & ~kResolved) : g0 + k;
                 roots += cur == g0 + k;
-                if (ckey && cur == g0 + k) ckey[g0 + k] = ~0u;   // canonical keys: global roots are tile-local roots
+                // first-voxel keys start at the root block's own first voxel
+                // (global roots are tile-local roots)
+                if (ckey && cur == g0 + k) ckey[g0 + k] = block_key3(h, w, bw, bz, by, bxs + k, m, voxel_key);
             }
-            const uint32_t m = ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
-                               (((e >> (2 * k)) & 3u) << 6);
             out[k] = m ? cur : g0 + k;
             mk[k >> 2] |= m << (8 * (k & 3));
         }
@@ -544,7 +558,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
 // canonical numbering.  The keys kernel also clears the key bits (c.nwords
 // words); the mark kernel then sets the bit of every root's key.
 
-template <bool VEC, bool VOXEL_KEY>
+template <bool VOXEL_KEY>
 __global__ __launch_bounds__(NT_FINAL) void bleq3d_keys(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
                                                          uint32_t h, uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
                                                          uint32_t xgroups, Canon c)
@@ -556,23 +570,38 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_keys(const uint32_t *__restri
     const uint32_t bxs = ((blockIdx.x % xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
     const bool inside = bz < bd && bxs < bw;
     const size_t b0 = ((size_t)bz * bh + by) * bw + bxs;
-    uint32_t pv[FINAL_BLOCKS], mv[FINAL_BLOCKS], root[FINAL_BLOCKS];
-    lane_blocks3<VEC>(P, M, b0, inside, bxs, bw, pv, mv, root);
+    uint32_t pv[FINAL_BLOCKS] = {}, mv[FINAL_BLOCKS] = {};
+    if (inside)
+        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k)
+            if (bxs + k < bw) {
+                pv[k] = P[b0 + k];
+                mv[k] = M[b0 + k];
+            }
 
-    const uint32_t plane = bh * bw, plane0 = bz * plane;
-    uint32_t r0 = ~0u, k0 = ~0u, r1 = ~0u, k1 = ~0u;
+    // Roots are never above parents (parent <= self), so only a block whose
+    // parent lies in this plane can have its root here: the others need no
+    // chase.  A root's key starts at its own block's key, so only offers below
+    // the current key are made.
+    const uint32_t plane0 = bz * (bh * bw);
+    uint32_t r0 = ~0u, k0 = ~0u, r1 = ~0u, k1 = ~0u, prev_p = ~0u, prev_r = 0, prev_k = 0;
 #pragma unroll
     for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
         const uint32_t m = mv[k];
-        if (!m || root[k] - plane0 >= plane) continue;   // not in its component's front plane
-        const uint32_t bit = __builtin_ctz(m), dx = bit & 1u, dy = (bit >> 1) & 1u, dz = bit >> 2;
-        const uint32_t zy = (2 * bz + dz) * h + 2 * by + dy;
-        const uint32_t key = VOXEL_KEY ? zy * w + 2 * (bxs + k) + dx : zy * bw + bxs + k;
-        if (r0 == ~0u || r0 == root[k]) {
-            r0 = root[k];
+        if (!m || pv[k] < plane0) continue;
+        if (pv[k] != prev_p) {
+            prev_p = pv[k];
+            prev_r = Forest::find_final(P, pv[k]);
+            prev_k = prev_r >= plane0 ? c.key[prev_r] : 0u;
+        }
+        const uint32_t r = prev_r;
+        if (r < plane0) continue;   // not in its component's front plane
+        const uint32_t key = block_key3(h, w, bw, bz, by, bxs + k, m, VOXEL_KEY);
+        if (key >= prev_k) continue;
+        if (r0 == ~0u || r0 == r) {
+            r0 = r;
             k0 = min(k0, key);
         } else {
-            r1 = root[k];
+            r1 = r;
             k1 = key;
         }
     }
@@ -638,18 +667,17 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
     constexpr uint32_t NT_TILE = Tile3<TX, TY, TZ>::THREADS;
     if (vec_in)
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
-                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key);
+                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
     else
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
-                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key);
+                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
     if (kt) kt->mark(s, "bleq3d_tile");
     hipLaunchKernelGGL(bleq3d_merge, dim3(merge_grid), dim3(MERGE_THREADS), 0, s, P, M, g, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_merge");
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
     if (keys) {
         // canonical numbering uses bit-position keys (the same for both paths); otherwise voxel indices
-        auto kern = canonical ? (vec_out ? bleq3d_keys<true, false> : bleq3d_keys<false, false>)
-                              : (vec_out ? bleq3d_keys<true, true> : bleq3d_keys<false, true>);
+        auto kern = canonical ? bleq3d_keys<false> : bleq3d_keys<true>;
         hipLaunchKernelGGL(kern, fgrid, dim3(NT_FINAL), 0, s, P, M, im.h, im.w, bd, bh, bw, xgroups, canon);
         if (kt) kt->mark(s, "bleq3d_keys");
     }
