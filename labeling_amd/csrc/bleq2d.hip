@@ -143,7 +143,10 @@ __device__ __forceinline__ uint32_t block_key(const Geom2D &G, uint32_t by, uint
 
 constexpr uint32_t LANE_BLOCKS = kLaneBlocks;
 
-template <uint32_t TBW, uint32_t TBH, class Src>
+// CONN4: 4-connectivity (the label equivalence path): blocks are 1x2 pixels and
+// a band is one pixel row; a row run is 4-connected, and only straight
+// contacts join rows.
+template <uint32_t TBW, uint32_t TBH, class Src, bool CONN4>
 __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src)
 {
     constexpr uint32_t LPB = TBW / LANE_BLOCKS;   // lanes per band
@@ -163,11 +166,11 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
 
     uint32_t a = 0, b = 0;
     if (by < bh && bxs < bw) {
-        const uint32_t y0 = 2 * by, x0 = 2 * bxs;
+        const uint32_t y0 = CONN4 ? by : 2 * by, x0 = 2 * bxs;
         const uint8_t *row0 = G.px + (size_t)y0 * G.pitch;
         const bool vec = (G.flags & kVecIn) != 0;
         a = row_bits32(row0, x0, w, vec);
-        if (y0 + 1 < h) b = row_bits32(row0 + G.pitch, x0, w, vec);
+        if (!CONN4 && y0 + 1 < h) b = row_bits32(row0 + G.pitch, x0, w, vec);
     }
     const uint32_t o = a | b;
     const uint32_t st = o & ~(o << 1);               // first column of each segment-local run
@@ -177,7 +180,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
     const uint32_t first = run_head(o, j, r * TBW);
 
     so[r + 1][j + 1] = o;
-    sb[r + 1][j + 1] = b;
+    sb[r + 1][j + 1] = CONN4 ? a : b;   // the row the band below touches
     sh[r + 1][j + 1] = first;
     if (j == 0) so[r + 1][0] = sb[r + 1][0] = 0;
     if (j == LPB - 1) so[r + 1][LPB + 1] = sb[r + 1][LPB + 1] = 0;
@@ -200,6 +203,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src
         const SegNodes up{ou & ~(ou << 1), nb - TBW, sh[r][j + 1], last_run_node(so[r][j], nb - TBW - LANE_BLOCKS, sh[r][j]),
                           sh[r][j + 2]};   // only used when a contact exists (r > 0)
         SegContacts cs(a, sb[r][j + 1], sb[r][j], sb[r][j + 2], st, up.st);
+        if (CONN4) cs.el = cs.er = 0;   // straight contacts only
         uint32_t ua = 0, ub = 0, oa = 0, ob = 0;
         bool busy = false;
         for (;;) {
@@ -369,7 +373,7 @@ __device__ __forceinline__ void band_rows(const uint32_t v[LANE_BLOCKS], uint32_
     }
 }
 
-template <uint32_t TBW, uint32_t TBH, class Src>
+template <uint32_t TBW, uint32_t TBH, class Src, bool CONN4>
 __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t total_tiles)
 {
     constexpr uint32_t LPB = TBW / LANE_BLOCKS;
@@ -417,10 +421,15 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
             uint32_t a, b, au, bu;
             band_rows(vo, a, b);
             band_rows(vu, au, bu);
-            const uint32_t bul = bxs > 0 ? (up[-1] & 8u) << 28 : 0u;                       // bottom-right pixel -> bit 31
-            const uint32_t bur = bxs + LANE_BLOCKS < bw ? (up[LANE_BLOCKS] >> 2) & 1u : 0u;   // bottom-left pixel -> bit 0
             const uint32_t o = a | b, ou = au | bu;
-            SegContacts cs(a, bu, bul, bur, o & ~(o << 1), ou & ~(ou << 1));
+            uint32_t bul = 0, bur = 0;
+            if (!CONN4) {
+                bul = bxs > 0 ? (up[-1] & 8u) << 28 : 0u;                       // bottom-right pixel -> bit 31
+                bur = bxs + LANE_BLOCKS < bw ? (up[LANE_BLOCKS] >> 2) & 1u : 0u;   // bottom-left pixel -> bit 0
+            }
+            // the band above touches with its bottom row (CONN4: its only row)
+            SegContacts cs(a, CONN4 ? au : bu, bul, bur, o & ~(o << 1), ou & ~(ou << 1));
+            if (CONN4) cs.el = cs.er = 0;
             while (cs.any()) {
                 uint32_t x;
                 const int p = cs.pop(x);
@@ -439,12 +448,16 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
                 // block bits: 1 top-left, 2 top-right, 4 bottom-left, 8 bottom-right
                 const uint32_t *w = G->P + (size_t)by * bw + bx0;
                 const uint32_t vo = w[0], m = vo & 15u;
-                if ((m & 5u) && (w[-1] & 10u)) push(vo >> 4, w[-1] >> 4);   // same band
-                if (r > 0 && (m & 1u)) {                                     // band above
+                if (CONN4) {
+                    if ((m & 1u) && (w[-1] & 2u)) push(vo >> 4, w[-1] >> 4);   // left neighbour pixel
+                } else if ((m & 5u) && (w[-1] & 10u)) {
+                    push(vo >> 4, w[-1] >> 4);                              // same band
+                }
+                if (!CONN4 && r > 0 && (m & 1u)) {                           // band above
                     const uint32_t vu = w[-(int64_t)bw - 1];
                     if (vu & 8u) push(vo >> 4, vu >> 4);
                 }
-                if (r + 1 < TBH && by + 1 < bh && (m & 4u)) {                // band below
+                if (!CONN4 && r + 1 < TBH && by + 1 < bh && (m & 4u)) {     // band below
                     const uint32_t vd = w[bw - 1];
                     if (vd & 2u) push(vo >> 4, vd >> 4);
                 }
@@ -573,7 +586,7 @@ __device__ __forceinline__ void lane_blocks(const Geom2D &G, uint32_t by, uint32
 // kModeFirst: 1 + the component's first pixel index (the label equivalence values).
 enum : uint32_t { kModeRoot = 0, kModeRef = 1, kModeCanon = 2, kModeFirst = 3, kModeAny = 4 };
 
-template <class Src, uint32_t MODE>
+template <class Src, uint32_t MODE, bool CONN4>
 __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
 {
     uint32_t lw;
@@ -597,16 +610,20 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
             if (r != prev_root) {
                 prev_root = r;
                 // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
-                prev_lab = canon ? canon_label(G.canon, G.canon.key[r])
-                                 : MODE == kModeFirst ? G.canon.key[r] + 1u
-                                                      : (ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u);
+                if (CONN4)   // the root block (1x2, the component's first) holds its first pixel
+                    prev_lab = canon ? canon_label(G.canon, r)
+                                     : 1u + (r / bw) * w + 2 * (r % bw) + ((G.P[r] & 1u) ? 0u : 1u);
+                else
+                    prev_lab = canon ? canon_label(G.canon, G.canon.key[r])
+                                     : MODE == kModeFirst ? G.canon.key[r] + 1u
+                                                          : (ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u);
             }
             lab[k] = prev_lab;
         }
-        const uint32_t y0 = 2 * by, x0 = 2 * bxs;
+        const uint32_t y0 = CONN4 ? by : 2 * by, x0 = 2 * bxs;
         uint32_t *row0 = G.labels + (size_t)y0 * w + x0;
         uint32_t *row1 = row0 + w;
-        const bool two_rows = y0 + 1 < h;
+        const bool two_rows = !CONN4 && y0 + 1 < h;
         uint32_t top[2 * FINAL_BLOCKS], bot[2 * FINAL_BLOCKS];
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
@@ -692,7 +709,9 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
     }
 }
 
-template <class Src>
+// CONN4: the key is the root block index itself (a 1x2 root block holds the
+// component's first pixel, so block order is first-pixel order).
+template <class Src, bool CONN4>
 __global__ __launch_bounds__(NT_FINAL) void bleq2d_mark(Src src)
 {
     uint32_t lw;
@@ -707,7 +726,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_mark(Src src)
         if (bxs + k >= bw) break;
         const uint32_t vb = G.P[b];
         if ((vb & 15u) && (vb >> 4) == b) {   // a root
-            const uint32_t key = G.canon.key[b];
+            const uint32_t key = CONN4 ? b : G.canon.key[b];
             if ((key >> 6) < G.canon.nwords) atomicOr(&G.canon.bits[key >> 6], 1ull << (key & 63u));
         }
     }
@@ -719,37 +738,42 @@ template <uint32_t TBW, uint32_t TBH>
 struct Variant {
     static constexpr uint32_t THREADS = (TBW / LANE_BLOCKS) * TBH;
 
-    template <class Src>
+    template <class Src, bool CONN4 = false>
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
                              KernelTimer *kt, bool flatten, const Canon *canon, uint32_t mode)
     {
         const uint32_t merge_grid =
             (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + MERGE_THREADS - 1) / MERGE_THREADS);
         const uint32_t border_grid = (uint32_t)(((uint64_t)total_tiles * Border<TBW, TBH>::JOBS + 255) / 256);
-        hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src>), dim3(total_tiles), dim3(THREADS), 0, s, src);
+        hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src, CONN4>), dim3(total_tiles), dim3(THREADS), 0, s, src);
         if (kt) kt->mark(s, "bleq2d_tile");
-        hipLaunchKernelGGL((bleq2d_merge<TBW, TBH, Src>), dim3(merge_grid), dim3(MERGE_THREADS), 0, s, src, total_tiles);
+        hipLaunchKernelGGL((bleq2d_merge<TBW, TBH, Src, CONN4>), dim3(merge_grid), dim3(MERGE_THREADS), 0, s, src,
+                           total_tiles);
         if (kt) kt->mark(s, "bleq2d_merge");
         if (flatten) {
             hipLaunchKernelGGL((bleq2d_flatten<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
             if (kt) kt->mark(s, "bleq2d_flatten");
         }
-        if (mode == kModeFirst || canon) {
-            hipLaunchKernelGGL((bleq2d_keys<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+        if (!CONN4 && (mode == kModeFirst || canon)) {
+            hipLaunchKernelGGL((bleq2d_keys<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);   // clears the bits
             if (kt) kt->mark(s, "bleq2d_keys");
         }
         if (canon) {
-            hipLaunchKernelGGL((bleq2d_mark<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+            if (CONN4) {
+                const hipError_t e = canon_clear_launch(*canon, s, kt);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL((bleq2d_mark<Src, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_mark");
             const hipError_t e = canon_scan_launch(*canon, s, kt);
             if (e != hipSuccess) return e;
         }
         switch (mode) {
-        case kModeRoot: hipLaunchKernelGGL((bleq2d_final<Src, kModeRoot>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        case kModeRef: hipLaunchKernelGGL((bleq2d_final<Src, kModeRef>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        case kModeCanon: hipLaunchKernelGGL((bleq2d_final<Src, kModeCanon>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        case kModeFirst: hipLaunchKernelGGL((bleq2d_final<Src, kModeFirst>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        default: hipLaunchKernelGGL((bleq2d_final<Src, kModeAny>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        case kModeRoot: hipLaunchKernelGGL((bleq2d_final<Src, kModeRoot, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        case kModeRef: hipLaunchKernelGGL((bleq2d_final<Src, kModeRef, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        case kModeCanon: hipLaunchKernelGGL((bleq2d_final<Src, kModeCanon, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        case kModeFirst: hipLaunchKernelGGL((bleq2d_final<Src, kModeFirst, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        default: hipLaunchKernelGGL((bleq2d_final<Src, kModeAny, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
         }
         if (kt) kt->mark(s, "bleq2d_final");
         return hipGetLastError();
@@ -827,6 +851,43 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
     if (kt) kt->begin(s);
     const uint32_t mode = canon ? kModeCanon : (first ? kModeFirst : ((g.flags & kRefLabels) ? kModeRef : kModeRoot));
     return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, canon ? &g.canon : nullptr, mode);
+}
+
+size_t bleq2d_conn4_workspace(uint32_t h, uint32_t w, bool canonical)
+{
+    const uint64_t bw = (w + 1ull) / 2;
+    return align256(kCountBytes) + align256(bw * h * sizeof(uint32_t)) + (canonical ? canon_workspace(bw * h, 0) : 0);
+}
+
+// Label equivalence with 4-connectivity on 1x2-pixel blocks: labels 1 + the
+// component's first pixel index (or canonical numbering).  The minimum block
+// of a component is its first block in raster order, on the component's first
+// row, so it holds the first pixel.
+hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
+{
+    constexpr uint32_t TBW = 128, TBH = 64;
+    One one;
+    Geom2D &g = one.g;
+    g = Geom2D{};
+    g.px = im.px;
+    g.pitch = im.pitch;
+    g.h = im.h;
+    g.w = im.w;
+    g.labels = im.labels;
+    g.count = im.count;
+    g.slots = static_cast<uint32_t *>(ws);
+    g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
+    bleq2d_geometry(g, 2);   // 128 x 64 blocks, then one pixel row per block row:
+    g.bh = g.h;
+    g.tiles_x = (g.bw + TBW - 1) / TBW;
+    g.ntiles = g.tiles_x * ((g.bh + TBH - 1) / TBH);
+    g.nfinal = g.xgroups * g.bh;
+    const bool canon = opt.mode == LBL_LABELS_CANONICAL;
+    if (canon) g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256((size_t)g.bw * g.bh * sizeof(uint32_t)),
+                                      (uint64_t)g.bw * g.bh, 0);
+    if (kt) kt->begin(s);
+    return Variant<TBW, TBH>::launch<One, true>(one, g.ntiles, g.nfinal, s, kt, false, canon ? &g.canon : nullptr,
+                                                canon ? kModeCanon : kModeFirst);
 }
 
 hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,

@@ -67,6 +67,12 @@ __global__ __launch_bounds__(kCanonThreads) void canon_prefix(Canon c)
     }
 }
 
+__global__ __launch_bounds__(kCanonThreads) void canon_clear(Canon c)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kCanonThreads + threadIdx.x; i < c.nwords; i += (uint64_t)gridDim.x * kCanonThreads)
+        c.bits[i] = 0ull;
+}
+
 uint64_t words_of(uint64_t nkeys) { return (nkeys + 63) / 64; }
 uint64_t chunks_of(uint64_t nwords) { return (nwords + kCanonChunkWords - 1) / kCanonChunkWords; }
 
@@ -93,6 +99,14 @@ Canon canon_layout(void *base, uint64_t nkeys, uint64_t nnodes)
     c.partial = reinterpret_cast<uint32_t *>(p);
     c.nwords = (uint32_t)nw;
     return c;
+}
+
+hipError_t canon_clear_launch(const Canon &c, hipStream_t s, KernelTimer *kt)
+{
+    const uint64_t blocks = (c.nwords + kCanonThreads - 1) / kCanonThreads;
+    hipLaunchKernelGGL(canon_clear, dim3((uint32_t)(blocks < 4096 ? blocks : 4096)), dim3(kCanonThreads), 0, s, c);
+    if (kt) kt->mark(s, "canon_clear");
+    return hipGetLastError();
 }
 
 hipError_t canon_scan_launch(const Canon &c, hipStream_t s, KernelTimer *kt)

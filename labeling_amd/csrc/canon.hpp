@@ -32,6 +32,9 @@ constexpr uint32_t kCanonChunkWords = kCanonThreads * kCanonWordsPerThread;   //
 size_t canon_workspace(uint64_t nkeys, uint64_t nnodes);
 Canon canon_layout(void *base, uint64_t nkeys, uint64_t nnodes);
 
+// Enqueue a fill of c.bits with zeros.
+hipError_t canon_clear_launch(const Canon &c, hipStream_t s, KernelTimer *kt);
+
 // Enqueue the scan over c.bits (after every bit is set): partial counts per
 // chunk, then prefix per word.
 hipError_t canon_scan_launch(const Canon &c, hipStream_t s, KernelTimer *kt);

@@ -296,7 +296,8 @@ __global__ __launch_bounds__(256) void lbeq_relabel(uint32_t *__restrict__ L, ui
 size_t lbeq2d_workspace(uint32_t h, uint32_t w, bool canonical)
 {
     const size_t own = align256(kCountBytes) + (canonical ? canon_workspace((uint64_t)h * w, 0) : 0);
-    return std::max(own, bleq2d_workspace(h, w, true));   // 8-connectivity runs on the block forest
+    // both connectivities run on block forests (below the block limit)
+    return std::max({own, bleq2d_workspace(h, w, true), bleq2d_conn4_workspace(h, w, canonical)});
 }
 
 // Shared with the 3D label-equivalence path: the analyze pass over n labels,
@@ -340,6 +341,10 @@ hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws,
     // the label equivalence values (1 + minimum pixel index)
     if (conn == 8 && ((im.h + 1ull) / 2) * ((im.w + 1ull) / 2) <= kMaxBlocks2D)
         return bleq2d_run(im, opt, ws, s, kt, true);
+    // 4-connectivity: 1x2 blocks, whose two pixels are 4-adjacent; the sweeps
+    // below remain for the iteration-cap debug knob and beyond the block limit
+    if (conn == 4 && opt.iter_cap == 0 && (uint64_t)im.h * ((im.w + 1ull) / 2) <= kMaxBlocks2D)
+        return bleq2d_run_conn4(im, opt, ws, s, kt);
     const uint32_t tiles_x = (im.w + TW - 1) / TW, tiles_y = (im.h + TH - 1) / TH;
     const uint32_t ntiles = tiles_x * tiles_y;
     const bool vec = (im.pitch % 16 == 0) && (reinterpret_cast<uintptr_t>(im.px) % 16 == 0);

@@ -78,6 +78,10 @@ size_t lbeq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical = fal
 // forest, for 8-connectivity (the pixels of a 2x2 block are 8-connected).
 hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt,
                       bool first = false);
+// Label equivalence values with 4-connectivity from the same kernels on 1x2
+// blocks (one pixel row per band), and its workspace.
+hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+size_t bleq2d_conn4_workspace(uint32_t h, uint32_t w, bool canonical);
 hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);

@@ -127,3 +127,19 @@ def test_full_size_conn8(labeler, name):
     c, m = O.canonical(lab)
     assert hashlib.sha256(c.tobytes()).hexdigest() == entry["conn8"]["labels_sha256"]
     assert n == m == entry["conn8"]["components"]
+
+
+@pytest.mark.parametrize("shape", [(257, 511), (1000, 100), (517, 1029)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_block_path_equals_sweep_path(labeler, shape):
+    """4-connectivity runs on 1x2 blocks; with an iteration cap set, it runs the
+    tile sweeps instead.  Both give the same raw labels."""
+    img = L.bernoulli(shape, 0.5, seed=99)
+    print(f"[lbeq2d block vs sweeps {shape}]", flush=True)
+    fast, n1, _ = labeler.label2d(img, "lbeq", conn=4)
+    labeler.set_iteration_cap(1 << 30)
+    try:
+        swept, n2, _ = labeler.label2d(img, "lbeq", conn=4)
+    finally:
+        labeler.set_iteration_cap(0)
+    np.testing.assert_array_equal(fast, swept)
+    assert n1 == n2
