@@ -146,8 +146,11 @@ constexpr uint32_t LANE_BLOCKS = kLaneBlocks;
 // CONN4: 4-connectivity (the label equivalence path): blocks are 1x2 pixels and
 // a band is one pixel row; a row run is 4-connected, and only straight
 // contacts join rows.
+// Single images: at most 64 VGPRs (8 waves per SIMD); the kernel needs 65
+// otherwise, which leaves three 512-lane workgroups per CU where LDS holds four
+// (4% faster).  The batch instances would spill under that limit.
 template <uint32_t TBW, uint32_t TBH, class Src, bool CONN4>
-__global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH) void bleq2d_tile(Src src)
+__global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 128) ? 8 : 1) void bleq2d_tile(Src src)
 {
     constexpr uint32_t LPB = TBW / LANE_BLOCKS;   // lanes per band
     static_assert((TBW & (TBW - 1)) == 0 && LPB >= 1 && LPB <= 64, "TBW: power of two, 16..1024");

@@ -75,17 +75,21 @@ private:
 struct Req {
     uint64_t ticket;
     lbl_request r;
+    lbl_label_mode mode = LBL_LABELS_ROOT;   // the context's label mode at submission
     double t_submit = 0, t_start = 0;
     bool is3d() const { return r.d != 0; }
-    bool batchable() const { return !is3d() && r.alg == LBL_BLEQ; }
+    bool canonical() const { return mode == LBL_LABELS_CANONICAL; }
+    // fused 2D bleq launches; canonical numbering runs per request
+    bool batchable() const { return !is3d() && r.alg == LBL_BLEQ && !canonical(); }
     uint64_t elems() const { return (uint64_t)(r.d ? r.d : 1) * r.h * r.w; }
     uint64_t in_bytes() const { return is3d() ? align256(elems()) : align256((uint64_t)r.h * ((r.w + 15u) & ~15u)); }
     uint64_t lab_bytes() const { return align256(elems() * 4); }
     uint64_t ws_bytes() const
     {
+        const bool c = canonical();
         if (is3d())
-            return align256(r.alg == LBL_BLEQ ? bleq3d_workspace(r.d, r.h, r.w) : lbeq3d_workspace(r.d, r.h, r.w));
-        return align256(r.alg == LBL_BLEQ ? bleq2d_workspace(r.h, r.w) : lbeq2d_workspace(r.h, r.w));
+            return align256(r.alg == LBL_BLEQ ? bleq3d_workspace(r.d, r.h, r.w, c) : lbeq3d_workspace(r.d, r.h, r.w, c));
+        return align256(r.alg == LBL_BLEQ ? bleq2d_workspace(r.h, r.w, c) : lbeq2d_workspace(r.h, r.w, c));
     }
     uint64_t bytes() const { return in_bytes() + lab_bytes() + ws_bytes(); }
 };
@@ -171,7 +175,7 @@ void lbl_server::launch(Group *g)
             gm.pitch = pitch;
             gm.h = q.h;
             gm.w = q.w;
-            gm.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
+            gm.flags = g->reqs[i]->mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
             bleq2d_geometry(gm, 0);
             gm.slots = reinterpret_cast<uint32_t *>(ws);   // count slots first, zeroed below
             (void)hipMemsetAsync(ws, 0, kCountBytes, s);
@@ -190,6 +194,7 @@ void lbl_server::launch(Group *g)
     } else {
         Req *rq = g->reqs[0];
         const lbl_request &q = rq->r;
+        opt.mode = rq->mode;
         uint8_t *d_in = base + cur;
         uint32_t *d_lab = reinterpret_cast<uint32_t *>(d_in + rq->in_bytes());
         void *ws = reinterpret_cast<uint8_t *>(d_lab) + rq->lab_bytes();
@@ -203,7 +208,8 @@ void lbl_server::launch(Group *g)
             const uint32_t pitch = (q.w + 15u) & ~15u;
             (void)upload2d(d_in, pitch, q.px, q.w, q.w, q.h, reinterpret_cast<uint8_t *>(d_lab), rq->lab_bytes(), s);
             Image2D im{d_in, pitch, q.h, q.w, d_lab, d_counts};
-            (void)lbeq2d_run(im, q.conn == 8 ? 8 : 4, opt, ws, s, nullptr);
+            if (q.alg == LBL_BLEQ) (void)bleq2d_run(im, opt, ws, s, nullptr);   // canonical numbering
+            else (void)lbeq2d_run(im, q.conn == 8 ? 8 : 4, opt, ws, s, nullptr);
         }
         (void)hipMemcpyAsync(q.labels, d_lab, rq->elems() * 4, hipMemcpyDeviceToHost, s);
     }
@@ -351,11 +357,11 @@ extern "C" lbl_status lbl_server_submit(lbl_server *srv, const lbl_request *req,
         return LBL_EUNSUPPORTED;
     }
     const lbl_label_mode mode = srv->ctx->opt.mode;
-    if (mode == LBL_LABELS_CANONICAL || (mode == LBL_LABELS_REFERENCE && (q.d != 0 || q.alg != LBL_BLEQ)))
-        return LBL_EUNSUPPORTED;
+    if (mode == LBL_LABELS_REFERENCE && (q.d != 0 || q.alg != LBL_BLEQ)) return LBL_EUNSUPPORTED;
     Req *r = new (std::nothrow) Req;
     if (!r) return LBL_ENOMEM;
     r->r = q;
+    r->mode = mode;
     if ((uint64_t)(q.d ? q.d : 1) * q.h * q.w > kMaxElems) {
         delete r;
         return LBL_EUNSUPPORTED;

@@ -89,3 +89,24 @@ def test_admission_limits(labeler):
         out.free()
     finally:
         srv.close()
+
+
+def test_canonical_mode_mix_equals_alone(labeler):
+    """In LBL_LABELS_CANONICAL mode the server labels every request with the
+    device canonical numbering (2D bleq requests are not fused then)."""
+    reqs = _mixed_requests(30, seed=9)
+    labeler.set_mode(L.LABELS_CANONICAL)
+    try:
+        print("[serve canonical]", flush=True)
+        got = _run(labeler, reqs, streams=3, hbm_budget_bytes=64 << 20, max_batch=4)
+        for k, (img, alg, conn) in enumerate(reqs):
+            if img.ndim == 2:
+                alone, n, _ = labeler.label2d(img, alg, conn=conn)
+            else:
+                alone, n, _ = labeler.label3d(img, alg)
+            lab, ncomp = got[k]
+            np.testing.assert_array_equal(lab, alone, err_msg=f"request {k} {alg} {img.shape}")
+            assert ncomp == n
+            assert lab.max() == n   # canonical: labels 1..n
+    finally:
+        labeler.set_mode(L.LABELS_ROOT)
