@@ -36,6 +36,8 @@
 //
 // HBM traffic per pixel: 1 B image in + 1 B block words out in (1), 1 B block
 // words in + 4 B labels out in (4), plus the border blocks in (2)-(3).
+#include <type_traits>
+
 #include "bleq2d.hpp"
 #include "uf_device.hpp"
 
@@ -381,12 +383,17 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
 {
     constexpr uint32_t LPB = TBW / LANE_BLOCKS;
     constexpr bool kSet = Src::kSingle;
-    __shared__ LdsPairSet<kSet ? HASH_CAP : 64> set;   // pairs of roots (single image)
-    __shared__ uint2 queue[kSet ? 1 : QUEUE_CAP];      // pairs of roots (batches)
-    __shared__ uint32_t qimg[kSet ? 1 : QUEUE_CAP];    // and their image
-    __shared__ uint32_t queued;
+    struct Queue {                 // batches: pairs of roots and their image
+        uint2 pair[QUEUE_CAP];
+        uint32_t img[QUEUE_CAP];
+        uint32_t n;
+    };
+    // one or the other: the single-image set takes 40 KB, four workgroups per CU
+    __shared__ std::conditional_t<kSet, LdsPairSet<HASH_CAP>, Queue> store;
+    auto &set = reinterpret_cast<LdsPairSet<HASH_CAP> &>(store);
+    auto &queue = reinterpret_cast<Queue &>(store);
     if constexpr (kSet) set.init();
-    if (threadIdx.x == 0) queued = 0;
+    else if (threadIdx.x == 0) queue.n = 0;
     __syncthreads();
 
     const uint32_t gid = blockIdx.x * MERGE_THREADS + threadIdx.x, n_top = total_tiles * LPB;
@@ -399,10 +406,10 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
         if constexpr (kSet) {
             if (set.insert(ra, rb)) return;
         } else {
-            const uint32_t i = atomicAdd(&queued, 1u);
+            const uint32_t i = atomicAdd(&queue.n, 1u);
             if (i < QUEUE_CAP) {
-                queue[i] = make_uint2(ra, rb);
-                qimg[i] = img;
+                queue.pair[i] = make_uint2(ra, rb);
+                queue.img[i] = img;
                 return;
             }
         }
@@ -477,11 +484,11 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
         }
         if (G0.count) count_links(G0.slots, links);   // kernel-uniform branch
     } else {
-        const uint32_t n = queued < QUEUE_CAP ? queued : QUEUE_CAP;
+        const uint32_t n = queue.n < QUEUE_CAP ? queue.n : QUEUE_CAP;
         if (links && G->count) atomicSub(&G->slots[(blockIdx.x % kCountSlots) * kCountStride], links);
         for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
-            const uint2 e = queue[i];
-            const Geom2D &Gi = src.image(qimg[i]);
+            const uint2 e = queue.pair[i];
+            const Geom2D &Gi = src.image(queue.img[i]);
             if (Forest::unite(Gi.P, e.x, e.y) && Gi.count) atomicSub(&Gi.slots[(blockIdx.x % kCountSlots) * kCountStride], 1u);
         }
     }

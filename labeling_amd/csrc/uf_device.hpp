@@ -318,14 +318,16 @@ __device__ __forceinline__ void count_clear(uint32_t *slots)
 // distinct pair once, spread evenly over its threads:
 //   init() by every thread, __syncthreads(), insert() by any thread,
 //   __syncthreads(), then pairs 0..size()-1.
-// insert() returns false when its probe sequence finds no room: the caller
-// handles that pair itself.
+// insert() returns false when its probe sequence finds no room, or the list of
+// filled slots is full (LIST = CAP - 2 entries, so that the whole set takes
+// exactly 10 * CAP bytes of LDS): the caller handles that pair itself.
 template <uint32_t CAP, uint32_t PROBES = 16>
 struct LdsPairSet {
     static_assert((CAP & (CAP - 1)) == 0 && CAP >= 64 && CAP <= 65536, "CAP: power of two, 64..65536");
     static constexpr unsigned long long kEmpty = ~0ull;
+    static constexpr uint32_t LIST = CAP - 2;
     unsigned long long key[CAP];
-    uint16_t filled[CAP];
+    uint16_t filled[LIST];
     uint32_t n;
 
     __device__ __forceinline__ void init()
@@ -337,14 +339,23 @@ struct LdsPairSet {
     {
         const unsigned long long k = (unsigned long long)a << 32 | b;
         uint32_t h = (a * 0x9E3779B1u ^ b * 0x85EBCA6Bu) >> (32 - __builtin_ctz(CAP));
+        bool placed = false;
         for (uint32_t t = 0; t < PROBES; ++t, h = (h + 1) & (CAP - 1)) {
             const unsigned long long old = atomicCAS(&key[h], kEmpty, k);
-            if (old == kEmpty) filled[atomicAdd(&n, 1u)] = (uint16_t)h;   // one entry per slot: < CAP
-            if (old == kEmpty || old == k) return true;
+            if (old == k) return true;   // already there
+            if (old == kEmpty) {
+                placed = true;
+                break;
+            }
         }
-        return false;
+        if (!placed) return false;
+        // first insertion of k: list it, or hand it back when the list is full
+        const uint32_t i = atomicAdd(&n, 1u);
+        if (i >= LIST) return false;
+        filled[i] = (uint16_t)h;
+        return true;
     }
-    __device__ __forceinline__ uint32_t size() const { return n; }
+    __device__ __forceinline__ uint32_t size() const { return n < LIST ? n : LIST; }
     __device__ __forceinline__ uint2 pair(uint32_t i) const
     {
         const unsigned long long k = key[filled[i]];
