@@ -64,9 +64,9 @@ typedef enum {
     LBL_LABELS_REFERENCE = 1,
     /* 1..N in raster order of each component's first pixel (voxel), x
      * fastest: the canonical form used for parity, computed on the device.
-     * Single images and volumes, both algorithms (lbl_label2d/3d[_dev],
-     * lbl_label2d_each, and the server, which then runs each request on its
-     * own); batches return LBL_EUNSUPPORTED.  Costs a
+     * Every entry point: single images and volumes of both algorithms,
+     * lbl_label2d_each, batches (per image), and the server (which then runs
+     * each request on its own).  Costs a
      * key pass over the block words plus a bit-array scan (bleq), or a relabel
      * pass over the labels (lbeq), and a larger workspace. */
     LBL_LABELS_CANONICAL = 2

@@ -18,6 +18,7 @@
 #include <functional>
 #include <new>
 #include <sstream>
+#include <type_traits>
 
 #include "bleq2d.hpp"
 #include "uf_device.hpp"
@@ -33,7 +34,8 @@ struct lbl_batch {
     std::vector<Geom2D> geoms;
     Geom2D *d_geoms = nullptr;
     void *ws = nullptr;
-    uint32_t total_tiles = 0, total_final = 0;
+    uint32_t total_tiles = 0, total_final = 0, total_chunks = 0;   // chunks: canonical scan (0: off)
+    bool canonical = false;
     uint32_t tile = 0;
     bool flatten = true;
     bool use_graph = true;
@@ -95,7 +97,7 @@ size_t plan(lbl_batch *b, uint32_t tile)
 {
     b->geoms.resize(b->n);
     uint64_t off = 0;
-    uint32_t tb = 0, fb = 0;
+    uint32_t tb = 0, fb = 0, cb = 0;
     for (uint32_t i = 0; i < b->n; ++i) {
         const lbl_image2d &im = b->images[i];
         Geom2D g{};
@@ -106,18 +108,27 @@ size_t plan(lbl_batch *b, uint32_t tile)
         g.h = im.h;
         g.w = im.w;
         g.flags = b->ctx->opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
+        if (b->canonical) g.flags |= kKeys | kCanonLabels;   // bit-position keys
         bleq2d_geometry(g, tile);
         g.tile_base = tb;
         g.final_base = fb;
+        g.chunk_base = cb;
         g.slots = reinterpret_cast<uint32_t *>(off);   // offsets, rebased once ws exists
         g.P = reinterpret_cast<uint32_t *>(off + align256(kCountBytes));
         tb += g.ntiles;
         fb += g.nfinal;
         off += align256(kCountBytes) + align256((uint64_t)g.bw * g.bh * sizeof(uint32_t));
+        if (b->canonical) {   // key array, bits, prefix and partial sums of this image
+            const uint64_t nkeys = (uint64_t)g.h * g.bw, nodes = (uint64_t)g.bw * g.bh;
+            g.canon = canon_layout(reinterpret_cast<void *>(off), nkeys, nodes);
+            cb += canon_chunks(g.canon.nwords);
+            off += canon_workspace(nkeys, nodes);
+        }
         b->geoms[i] = g;
     }
     b->total_tiles = tb;
     b->total_final = fb;
+    b->total_chunks = cb;
     b->tile = tile;
     return (size_t)off;
 }
@@ -126,9 +137,21 @@ lbl_status upload(lbl_batch *b)
 {
     lbl_ctx *ctx = b->ctx;
     std::vector<Geom2D> dev = b->geoms;
+    uint8_t *base = static_cast<uint8_t *>(b->ws);
+    // every planned pointer is an offset into the workspace (the first image's
+    // count slots are at offset 0, so a null pointer here is offset 0)
+    const auto rebase = [base](auto *&p) {
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + reinterpret_cast<uintptr_t>(p));
+    };
     for (auto &g : dev) {
-        g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(b->ws) + reinterpret_cast<uintptr_t>(g.P));
-        g.slots = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(b->ws) + reinterpret_cast<uintptr_t>(g.slots));
+        rebase(g.P);
+        rebase(g.slots);
+        if (b->canonical) {
+            rebase(g.canon.key);
+            rebase(g.canon.bits);
+            rebase(g.canon.prefix);
+            rebase(g.canon.partial);
+        }
     }
     B_HIP(ctx, hipMemcpyAsync(b->d_geoms, dev.data(), dev.size() * sizeof(Geom2D), hipMemcpyHostToDevice, ctx->stream));
     B_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -137,7 +160,8 @@ lbl_status upload(lbl_batch *b)
 
 lbl_status launch(lbl_batch *b, hipStream_t s)
 {
-    hipError_t e = bleq2d_run_batch(b->d_geoms, b->n, b->total_tiles, b->total_final, b->tile, b->flatten, s, nullptr);
+    hipError_t e = bleq2d_run_batch(b->d_geoms, b->n, b->total_tiles, b->total_final, b->total_chunks, b->tile,
+                                    b->flatten, s, nullptr);
     if (e != hipSuccess) return bhip(b->ctx, "bleq2d batch launch", e);
     return LBL_OK;
 }
@@ -149,7 +173,8 @@ lbl_status build_graph(lbl_batch *b)
     B_HIP(ctx, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t le = bleq2d_run_batch(b->d_geoms, b->n, b->total_tiles, b->total_final, b->tile, b->flatten, cs, nullptr);
+        hipError_t le = bleq2d_run_batch(b->d_geoms, b->n, b->total_tiles, b->total_final, b->total_chunks, b->tile,
+                                         b->flatten, cs, nullptr);
         e = hipStreamEndCapture(cs, &b->graph);
         if (le != hipSuccess) e = le;
     }
@@ -214,8 +239,6 @@ extern "C" lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
     if (alg != LBL_BLEQ) return bfail(ctx, LBL_EINVAL, "unknown algorithm id");
     if (conn == 4) return bfail(ctx, LBL_EUNSUPPORTED, "bleq: method does not support 4-connectivity");
     if (conn != 0 && conn != 8) return bfail(ctx, LBL_EINVAL, "connectivity must be 0, 4 or 8 for 2D");
-    if (ctx->opt.mode == LBL_LABELS_CANONICAL)
-        return bfail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_CANONICAL is not available in this build");
     for (uint32_t i = 0; i < n; ++i) {
         const lbl_image2d &im = images[i];
         if (!im.px || !im.labels || im.h == 0 || im.w == 0 || im.pitch_bytes < im.w)
@@ -229,6 +252,7 @@ extern "C" lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
     b->ctx = ctx;
     b->alg = alg;
     b->n = n;
+    b->canonical = ctx->opt.mode == LBL_LABELS_CANONICAL;
     b->images.assign(images, images + n);
     b->d_ncomp = d_ncomp;
     b->use_graph = !(flags & LBL_BATCH_NO_GRAPH);

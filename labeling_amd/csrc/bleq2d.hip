@@ -76,6 +76,11 @@ struct One {
         lw = gw;
         return g;
     }
+    __device__ __forceinline__ const Geom2D &by_chunk(uint32_t gc, uint32_t &lc) const
+    {
+        lc = gc;
+        return g;
+    }
 };
 
 struct Many {
@@ -88,7 +93,7 @@ struct Many {
         uint32_t lo = 0, hi = n;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            const uint32_t base = FIELD == 0 ? g[mid].tile_base : g[mid].final_base;
+            const uint32_t base = FIELD == 0 ? g[mid].tile_base : (FIELD == 1 ? g[mid].final_base : g[mid].chunk_base);
             if (base <= id) lo = mid;
             else hi = mid;
         }
@@ -107,6 +112,12 @@ struct Many {
     {
         const Geom2D &r = g[find<1>(gw)];
         lw = gw - r.final_base;
+        return r;
+    }
+    __device__ __forceinline__ const Geom2D &by_chunk(uint32_t gc, uint32_t &lc) const
+    {
+        const Geom2D &r = g[find<2>(gc)];
+        lc = gc - r.chunk_base;
         return r;
     }
 };
@@ -610,7 +621,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
     if (by < G.bh && bxs < bw) {
         uint32_t lab[FINAL_BLOCKS];
         const bool ref = MODE == kModeRef || (MODE == kModeAny && (G.flags & kRefLabels));
-        const bool canon = MODE == kModeCanon;
+        const bool canon = MODE == kModeCanon || (MODE == kModeAny && (G.flags & kCanonLabels));
         uint32_t prev_root = ~0u, prev_lab = 0;
 #pragma unroll
         for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
@@ -742,6 +753,31 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_mark(Src src)
     }
 }
 
+// The canonical scan (canon.hpp) over every image's chunks: workgroup = chunk.
+template <class Src>
+__global__ __launch_bounds__(kCanonThreads) void bleq2d_canon_clear(Src src)
+{
+    uint32_t lc;
+    const Geom2D &G = src.by_chunk(blockIdx.x, lc);
+    canon_clear_chunk(G.canon, lc);
+}
+
+template <class Src>
+__global__ __launch_bounds__(kCanonThreads) void bleq2d_canon_partial(Src src)
+{
+    uint32_t lc;
+    const Geom2D &G = src.by_chunk(blockIdx.x, lc);
+    canon_partial_chunk(G.canon, lc);
+}
+
+template <class Src>
+__global__ __launch_bounds__(kCanonThreads) void bleq2d_canon_prefix(Src src)
+{
+    uint32_t lc;
+    const Geom2D &G = src.by_chunk(blockIdx.x, lc);
+    canon_prefix_chunk(G.canon, lc);
+}
+
 // ---- host side --------------------------------------------------------------------
 
 template <uint32_t TBW, uint32_t TBH>
@@ -750,8 +786,9 @@ struct Variant {
 
     template <class Src, bool CONN4 = false>
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
-                             KernelTimer *kt, bool flatten, const Canon *canon, uint32_t mode)
+                             KernelTimer *kt, bool flatten, uint32_t canon_chunks, uint32_t mode)
     {
+        const bool canon = canon_chunks != 0;   // some image asks for canonical labels
         const uint32_t merge_grid =
             (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + MERGE_THREADS - 1) / MERGE_THREADS);
         const uint32_t border_grid = (uint32_t)(((uint64_t)total_tiles * Border<TBW, TBH>::JOBS + 255) / 256);
@@ -769,14 +806,16 @@ struct Variant {
             if (kt) kt->mark(s, "bleq2d_keys");
         }
         if (canon) {
-            if (CONN4) {
-                const hipError_t e = canon_clear_launch(*canon, s, kt);
-                if (e != hipSuccess) return e;
+            if (CONN4) {   // no keys pass to clear the bits
+                hipLaunchKernelGGL((bleq2d_canon_clear<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
+                if (kt) kt->mark(s, "canon_clear");
             }
             hipLaunchKernelGGL((bleq2d_mark<Src, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_mark");
-            const hipError_t e = canon_scan_launch(*canon, s, kt);
-            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((bleq2d_canon_partial<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
+            if (kt) kt->mark(s, "canon_partial");
+            hipLaunchKernelGGL((bleq2d_canon_prefix<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
+            if (kt) kt->mark(s, "canon_prefix");
         }
         switch (mode) {
         case kModeRoot: hipLaunchKernelGGL((bleq2d_final<Src, kModeRoot, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
@@ -790,20 +829,20 @@ struct Variant {
     }
 };
 
-// canon: the single image's canonical state when it asks for canonical labels;
-// mode: the label values (kMode*)
+// canon_chunks: canonical scan chunks of the images that ask for canonical
+// labels (0: none); mode: the label values (kMode*)
 template <class Src>
 hipError_t dispatch(uint32_t tile_id, const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
-                    KernelTimer *kt, bool flatten, const Canon *canon, uint32_t mode)
+                    KernelTimer *kt, bool flatten, uint32_t canon_chunks, uint32_t mode)
 {
     switch (tile_id) {
-    case 1: return Variant<64, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
-    case 2: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
-    case 3: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
-    case 4: return Variant<256, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
-    case 5: return Variant<512, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
-    case 6: return Variant<128, 128>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
-    default: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon, mode);
+    case 1: return Variant<64, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon_chunks, mode);
+    case 2: return Variant<128, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon_chunks, mode);
+    case 3: return Variant<256, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon_chunks, mode);
+    case 4: return Variant<256, 64>::launch(src, total_tiles, total_final, s, kt, flatten, canon_chunks, mode);
+    case 5: return Variant<512, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon_chunks, mode);
+    case 6: return Variant<128, 128>::launch(src, total_tiles, total_final, s, kt, flatten, canon_chunks, mode);
+    default: return Variant<128, 32>::launch(src, total_tiles, total_final, s, kt, flatten, canon_chunks, mode);
     }
 }
 
@@ -822,7 +861,7 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
     g.xgroups = (g.bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
     g.nfinal = g.xgroups * g.bh;
     g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
-    g.flags &= kRefLabels | kKeys | kPixelKeys;
+    g.flags &= kRefLabels | kKeys | kPixelKeys | kCanonLabels;
     if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
     if (g.w % 8 == 0 && reinterpret_cast<uintptr_t>(g.labels) % 16 == 0) g.flags |= kVecOut;
 }
@@ -860,7 +899,8 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
     }
     if (kt) kt->begin(s);
     const uint32_t mode = canon ? kModeCanon : (first ? kModeFirst : ((g.flags & kRefLabels) ? kModeRef : kModeRoot));
-    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, canon ? &g.canon : nullptr, mode);
+    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, canon ? canon_chunks(g.canon.nwords) : 0u,
+                    mode);
 }
 
 size_t bleq2d_conn4_workspace(uint32_t h, uint32_t w, bool canonical)
@@ -896,16 +936,16 @@ hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hip
     if (canon) g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256((size_t)g.bw * g.bh * sizeof(uint32_t)),
                                       (uint64_t)g.bw * g.bh, 0);
     if (kt) kt->begin(s);
-    return Variant<TBW, TBH>::launch<One, true>(one, g.ntiles, g.nfinal, s, kt, false, canon ? &g.canon : nullptr,
-                                                canon ? kModeCanon : kModeFirst);
+    return Variant<TBW, TBH>::launch<One, true>(one, g.ntiles, g.nfinal, s, kt, false,
+                                                canon ? canon_chunks(g.canon.nwords) : 0u, canon ? kModeCanon : kModeFirst);
 }
 
 hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,
-                            uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt)
+                            uint32_t total_chunks, uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt)
 {
     Many many{d_geoms, n};
     if (kt) kt->begin(s);
-    return dispatch(tile_id, many, total_tiles, total_final, s, kt, flatten, nullptr, kModeAny);
+    return dispatch(tile_id, many, total_tiles, total_final, s, kt, flatten, total_chunks, kModeAny);
 }
 
 }  // namespace lbl

@@ -12,6 +12,7 @@ enum : uint32_t {
     kRefLabels = 4u,   // write the reference's raw label values
     kKeys = 8u,        // keep first-pixel keys of roots (canonical or label equivalence values)
     kPixelKeys = 16u,  // keys are pixel indices (else canonical bit positions)
+    kCanonLabels = 32u,  // write canonical labels (batches: per image)
 };
 
 // One image as the kernels see it (also the element of a batch's device table).
@@ -27,6 +28,7 @@ struct Geom2D {
     uint32_t xgroups, nfinal;   // final-kernel workgroups: xgroups per block row, nfinal total
     uint32_t tile_base;         // first global tile id of this image (batch)
     uint32_t final_base;        // first global final workgroup id (batch)
+    uint32_t chunk_base;        // first global canonical scan chunk id (batch)
     uint32_t flags;
     uint32_t ref_bw;            // block-grid width of the reference's padded image
     Canon canon;                // canonical numbering state (kCanon)
@@ -43,7 +45,8 @@ extern const TileShape2D kBleq2dTiles[kBleq2dTileCount];
 // h, w and the kRefLabels / kCanon flags must be set).
 void bleq2d_geometry(Geom2D &g, uint32_t tile_id);
 
+// total_chunks: the images' canonical scan chunks (0: no image asks for canonical labels)
 hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,
-                            uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt);
+                            uint32_t total_chunks, uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt);
 
 }  // namespace lbl

@@ -49,4 +49,80 @@ __device__ __forceinline__ uint32_t canon_label(const Canon &c, uint32_t k)
     return 1u + c.prefix[w] + (uint32_t)__popcll(below);
 }
 
+// ---- the scan, per chunk of kCanonChunkWords words (one 256-thread workgroup) ----
+
+// Inclusive sum over the 256 threads of a workgroup; `total` gets the sum of all.
+__device__ __forceinline__ uint32_t canon_block_sum(uint32_t v, uint32_t &total)
+{
+    __shared__ uint32_t wave_sums[kCanonThreads / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    if (lane == 63) wave_sums[wave] = v;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kCanonThreads / 64; ++i) {
+        const uint32_t s = wave_sums[i];
+        before += i < wave ? s : 0u;
+        total += s;
+    }
+    __syncthreads();   // wave_sums reusable
+    return v + before;
+}
+
+// Set bits of this thread's kCanonWordsPerThread words of a chunk.
+__device__ __forceinline__ uint32_t canon_thread_bits(const Canon &c, uint32_t chunk,
+                                                      uint32_t counts[kCanonWordsPerThread])
+{
+    const uint64_t w0 = (uint64_t)chunk * kCanonChunkWords + (uint64_t)threadIdx.x * kCanonWordsPerThread;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kCanonWordsPerThread; ++i) {
+        counts[i] = w0 + i < c.nwords ? (uint32_t)__popcll(c.bits[w0 + i]) : 0u;
+        sum += counts[i];
+    }
+    return sum;
+}
+
+// partial[chunk] = set bits in the chunk (all 256 threads call it).
+__device__ __forceinline__ void canon_partial_chunk(const Canon &c, uint32_t chunk)
+{
+    uint32_t counts[kCanonWordsPerThread], total;
+    canon_block_sum(canon_thread_bits(c, chunk, counts), total);
+    if (threadIdx.x == 0) c.partial[chunk] = total;
+}
+
+// prefix[w] for the chunk's words, after every partial is written.
+__device__ __forceinline__ void canon_prefix_chunk(const Canon &c, uint32_t chunk)
+{
+    uint32_t base = 0, total;
+    for (uint32_t i = threadIdx.x; i < chunk; i += kCanonThreads) base += c.partial[i];   // chunks before
+    canon_block_sum(base, total);
+    base = total;
+    uint32_t counts[kCanonWordsPerThread];
+    const uint32_t mine = canon_thread_bits(c, chunk, counts);
+    uint32_t run = base + canon_block_sum(mine, total) - mine;
+    const uint64_t w0 = (uint64_t)chunk * kCanonChunkWords + (uint64_t)threadIdx.x * kCanonWordsPerThread;
+#pragma unroll
+    for (uint32_t i = 0; i < kCanonWordsPerThread; ++i) {
+        if (w0 + i < c.nwords) c.prefix[w0 + i] = run;
+        run += counts[i];
+    }
+}
+
+// Zero the chunk's words of the bit array.
+__device__ __forceinline__ void canon_clear_chunk(const Canon &c, uint32_t chunk)
+{
+    const uint64_t w0 = (uint64_t)chunk * kCanonChunkWords;
+    for (uint32_t i = threadIdx.x; i < kCanonChunkWords; i += kCanonThreads)
+        if (w0 + i < c.nwords) c.bits[w0 + i] = 0ull;
+}
+
+inline uint32_t canon_chunks(uint32_t nwords) { return (nwords + kCanonChunkWords - 1) / kCanonChunkWords; }
+
 }  // namespace lbl

@@ -186,7 +186,7 @@ void lbl_server::launch(Group *g)
         }
         Geom2D *d_geoms = reinterpret_cast<Geom2D *>(base);
         (void)hipMemcpyAsync(d_geoms, sl.h_geoms, sizeof(Geom2D) * n, hipMemcpyHostToDevice, s);
-        (void)bleq2d_run_batch(d_geoms, n, tb, fb, 0, true, s, nullptr);
+        (void)bleq2d_run_batch(d_geoms, n, tb, fb, 0, 0, true, s, nullptr);
         for (uint32_t i = 0; i < n; ++i) {
             const lbl_request &q = g->reqs[i]->r;
             (void)hipMemcpyAsync(q.labels, sl.h_geoms[i].labels, (size_t)q.h * q.w * 4, hipMemcpyDeviceToHost, s);

@@ -117,3 +117,38 @@ def test_batch_of_mixed_large_images(labeler):
             assert counts[i] == m
     finally:
         lib.lbl_batch_destroy(b)
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2], ids=["graph", "autotune", "eager"])
+def test_batch_canonical_labels(labeler, flags):
+    """LBL_LABELS_CANONICAL on a batch: every image's labels are its own
+    canonical numbering (1..n in raster order of first pixels), the oracle's,
+    on every replay of the captured graph."""
+    lib = _declare(labeler.lib)
+    imgs = _images(97, seed=23) + [np.zeros((5, 7), np.uint8), np.full((9, 4), 255, np.uint8)]
+    bufs, descs = _upload(imgs, pitch_pad=flags)
+    d_n = hiprt.DeviceBuffer(4 * len(imgs))
+    b = ctypes.c_void_p()
+    labeler.set_mode(L.LABELS_CANONICAL)
+    try:
+        st = lib.lbl_batch_create(labeler.handle, L.BLEQ, descs, len(imgs), 0, d_n.addr, flags, ctypes.byref(b))
+        assert st == L.OK, labeler.lib.lbl_last_error(labeler.handle)
+        print(f"[batch canonical flags={flags}]", flush=True)
+        for rep in range(3):
+            for _, d_lab in bufs:
+                d_lab.zero()
+            assert lib.lbl_batch_run(b, None) == L.OK
+            hiprt.synchronize()
+            counts = d_n.download((len(imgs),), np.uint32)
+            for i, (img, (_, d_lab)) in enumerate(zip(imgs, bufs)):
+                got = d_lab.download(img.shape, np.uint32)
+                expect, m = O.bfs2d(img, 8)
+                np.testing.assert_array_equal(got, expect, err_msg=f"image {i} {img.shape} replay {rep}")
+                assert counts[i] == m
+                if rep == 0 and i % 10 == 0:
+                    single, n, _ = labeler.label2d(img, "bleq")
+                    np.testing.assert_array_equal(single, got)
+    finally:
+        labeler.set_mode(L.LABELS_ROOT)
+        if b.value:
+            lib.lbl_batch_destroy(b)
