@@ -217,10 +217,11 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
                 ob = ub = nn.node(p);
                 busy = true;
             }
-            if (!lds_union_step(par, ua, ub)) continue;
-            // both original nodes now point straight at the set's root (or lower)
-            lds_min(&par[oa], ua);
-            lds_min(&par[ob], ua);
+            uint32_t top;
+            if (!lds_rem_step(par, ua, ub, top)) continue;
+            // both original nodes now point at the node where the sets met (or lower)
+            lds_min(&par[oa], top);
+            lds_min(&par[ob], top);
             busy = false;
         }
     }

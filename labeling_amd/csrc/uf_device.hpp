@@ -419,28 +419,35 @@ __device__ __forceinline__ uint32_t run_head(uint32_t o, uint32_t j, uint32_t ro
     return !below ? row0 : ((o_k >> 31) ? sk + (hibit(o_k & ~(o_k << 1)) >> 1) : sk + kLaneBlocks);
 }
 
-// One step of uniting the sets of ua and ub: while either is not a root both
-// hop once; two different roots are linked, the larger under the smaller (a
-// failed link continues from the parent that displaced it).  Returns true once
-// united, with ua the root of the union (or an ancestor of it).
-__device__ __forceinline__ bool lds_union_step(uint32_t *par, uint32_t &ua, uint32_t &ub)
+// One step of uniting the sets of x and y by Rem's algorithm with splicing:
+// of the two nodes, the one with the larger parent takes the other's parent
+// (atomicMin keeps parent <= self; it moves the node, with its subtree, into
+// the other set) and continues from its old parent, until both have the same
+// parent or a root is linked.  It stops short of the roots when the sets are
+// already one, which most contacts of a dense image are.  Returns true when
+// done, with `top` the node where the sets met (an ancestor of both).
+__device__ __forceinline__ bool lds_rem_step(uint32_t *par, uint32_t &x, uint32_t &y, uint32_t &top)
 {
-    const uint32_t pa = lds_load(&par[ua]), pb = lds_load(&par[ub]);
-    if (pa != ua || pb != ub) {
-        ua = pa;
-        ub = pb;
-        return false;
+    uint32_t px = lds_load(&par[x]), py = lds_load(&par[y]);
+    if (px == py) {
+        top = px;
+        return true;
     }
-    if (ua == ub) return true;
-    const uint32_t hi = ua > ub ? ua : ub, lo = ua ^ ub ^ hi;
-    const uint32_t old = lds_min(&par[hi], lo);
-    if (old != hi) {
-        ua = old;
-        ub = lo;
-        return false;
+    if (px < py) {
+        const uint32_t t = x;
+        x = y;
+        y = t;
+        const uint32_t tp = px;
+        px = py;
+        py = tp;
     }
-    ua = lo;
-    return true;
+    const uint32_t old = lds_min(&par[x], py);
+    if (old == x) {   // x was a root: now under py
+        top = py;
+        return true;
+    }
+    x = old;   // x's old parent (or whatever lowered it meanwhile) still has to join
+    return false;
 }
 
 // Contacts between a segment's facing row A and the facing row B of a
