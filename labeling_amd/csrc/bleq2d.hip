@@ -269,20 +269,26 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
     // components are counted at their roots (the merge subtracts its links)
     uint32_t roots = 0;
     if (by < bh && bxs < bw) {
-        uint32_t out[LANE_BLOCKS], cur = 0;
+        uint32_t out[LANE_BLOCKS], cur = 0, rootmask = 0;
 #pragma unroll
-        for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {
+        for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {   // selects only: no per-block branch
             const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
-            if (st & (3u << (2 * kk))) {
-                cur = (v[kk] & kResolved) ? v[kk] : g0 + kk;
-                const bool root = (cur & ~kResolved) == g0 + kk;
-                roots += root;
-                // first-pixel keys start at the root block's own first pixel
-                // (global roots are tile-local roots)
-                if (root && (G.flags & kKeys)) G.canon.key[g0 + kk] = block_key(G, by, bxs + kk, m);
-            }
-            out[kk] = m ? (cur << 4) | m : (g0 + kk) << 4;   // << 4 drops the tag bit
+            const bool node = (st & (3u << (2 * kk))) != 0;
+            const uint32_t own = g0 + kk;
+            const uint32_t nv = (v[kk] & kResolved) ? v[kk] : own;
+            cur = node ? nv : cur;
+            rootmask |= (node && nv == own) ? 1u << kk : 0u;
+            out[kk] = m ? (cur << 4) | m : own << 4;   // << 4 drops the tag bit
         }
+        roots = __builtin_popcount(rootmask);
+        // first-pixel keys start at the root block's own first pixel (global
+        // roots are tile-local roots)
+        if (G.flags & kKeys)
+            for (uint32_t rm = rootmask; rm; rm &= rm - 1) {
+                const uint32_t kk = __builtin_ctz(rm);
+                const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
+                G.canon.key[g0 + kk] = block_key(G, by, bxs + kk, m);
+            }
         uint32_t *dst = G.P + (size_t)by * bw + bxs;
         if ((bw % 4u) == 0 && bxs + LANE_BLOCKS <= bw) {
             uint4 *d4 = reinterpret_cast<uint4 *>(dst);
