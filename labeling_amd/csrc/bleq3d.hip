@@ -256,21 +256,30 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     // the tile's components are counted at their roots (the merge subtracts its links)
     uint32_t roots = 0;
     if (bz < bd && by < bh && bxs < bw) {
-        uint32_t out[kLaneBlocks], mk[kLaneBlocks / 4] = {0, 0, 0, 0}, cur = 0;
+        uint32_t out[kLaneBlocks], mk[kLaneBlocks / 4] = {0, 0, 0, 0}, cur = 0, rootmask = 0;
+        const auto mask_of = [&](uint32_t k) {
+            return ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
+                   (((e >> (2 * k)) & 3u) << 6);
+        };
 #pragma unroll
-        for (uint32_t k = 0; k < kLaneBlocks; ++k) {
-            const uint32_t m = ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
-                               (((e >> (2 * k)) & 3u) << 6);
-            if (st & (3u << (2 * k))) {
-                cur = (v[k] & kResolved) ? (v[k] & ~kResolved) : g0 + k;
-                roots += cur == g0 + k;
-                // first-voxel keys start at the root block's own first voxel
-                // (global roots are tile-local roots)
-                if (ckey && cur == g0 + k) ckey[g0 + k] = block_key3(h, w, bw, bz, by, bxs + k, m, voxel_key);
-            }
-            out[k] = m ? cur : g0 + k;
+        for (uint32_t k = 0; k < kLaneBlocks; ++k) {   // selects only: no per-block branch
+            const uint32_t m = mask_of(k);
+            const bool node = (st & (3u << (2 * k))) != 0;
+            const uint32_t own = g0 + k;
+            const uint32_t nv = (v[k] & kResolved) ? (v[k] & ~kResolved) : own;
+            cur = node ? nv : cur;
+            rootmask |= (node && nv == own) ? 1u << k : 0u;
+            out[k] = m ? cur : own;
             mk[k >> 2] |= m << (8 * (k & 3));
         }
+        roots = __builtin_popcount(rootmask);
+        // first-voxel keys start at the root block's own first voxel (global
+        // roots are tile-local roots)
+        if (ckey)
+            for (uint32_t rm = rootmask; rm; rm &= rm - 1) {
+                const uint32_t k = __builtin_ctz(rm);
+                ckey[g0 + k] = block_key3(h, w, bw, bz, by, bxs + k, mask_of(k), voxel_key);
+            }
         const size_t row = ((size_t)bz * bh + by) * bw + bxs;
         uint32_t *dst = P + row;
         uint8_t *mdst = M + row;

@@ -142,52 +142,21 @@ struct PackedForest {
         return x;
     }
 
-    // Unite the sets of x and y by Rem's algorithm with splicing: of the two
-    // nodes, the one with the larger parent takes the other's parent (an
-    // atomicMin that keeps the low field; it moves the node, with its subtree,
-    // into the other set) and continues from its old parent, until both have
-    // the same parent or a root is linked.  Most unions of a dense image meet
-    // long before the roots.  The two starting nodes are then pointed at the
-    // root above the meeting node, so that later finds from them (the final
-    // kernel's, from every block of their tiles) take one hop.  True iff this
-    // call made the link that joined two different sets (exactly one call per
-    // merge of two sets returns true: the number of roots drops by one per
-    // root link and by nothing else).
-    __device__ static bool unite(uint32_t *w, uint32_t x, uint32_t y)
+    // Unite the sets of a and b; true iff this call made the link that joined
+    // two different sets (exactly one call per merge of two sets returns true).
+    __device__ static bool unite(uint32_t *w, uint32_t a, uint32_t b)
     {
-        const uint32_t x0 = x, y0 = y;
-        const uint32_t v0x = dev_load(&w[x0]), v0y = dev_load(&w[y0]);
-        uint32_t vx = v0x, vy = v0y, top;
-        bool linked = false;
         for (;;) {
-            uint32_t px = vx >> SHIFT, py = vy >> SHIFT, lowx = vx & kLowMask;
-            if (px == py) {
-                top = px;
-                break;
-            }
-            if (px < py) {
-                const uint32_t t = x;
-                x = y;
-                y = t;
-                const uint32_t tp = px;
-                px = py;
-                py = tp;
-                lowx = vy & kLowMask;
-            }
-            const uint32_t old = dev_min(&w[x], (py << SHIFT) | lowx);
-            if ((old >> SHIFT) == x) {   // x was a root: now under py
-                top = py;
-                linked = true;
-                break;
-            }
-            x = old >> SHIFT;   // x's old parent (or whatever lowered it meanwhile) still has to join
-            vx = dev_load(&w[x]);
-            vy = dev_load(&w[y]);
+            a = find(w, a);
+            b = find(w, b);
+            if (a == b) return false;
+            if (a < b) { uint32_t t = a; a = b; b = t; }
+            uint32_t va = dev_load(&w[a]);
+            if ((va >> SHIFT) != a) continue;   // a was linked meanwhile: re-find
+            uint32_t old = dev_min(&w[a], (b << SHIFT) | (va & kLowMask));
+            if ((old >> SHIFT) == a) return true;
+            a = old >> SHIFT;
         }
-        for (uint32_t v = dev_load(&w[top]); (v >> SHIFT) != top; v = dev_load(&w[top])) top = v >> SHIFT;
-        if ((v0x >> SHIFT) > top) dev_min(&w[x0], (top << SHIFT) | (v0x & kLowMask));
-        if ((v0y >> SHIFT) > top) dev_min(&w[y0], (top << SHIFT) | (v0y & kLowMask));
-        return linked;
     }
 };
 

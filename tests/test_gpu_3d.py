@@ -114,3 +114,29 @@ def test_full_size_vs_scipy_golden(labeler, name, alg):
     c, m = O.canonical(lab)
     assert hashlib.sha256(c.tobytes()).hexdigest() == entry["conn26"]["labels_sha256"]
     assert n == m == entry["conn26"]["components"]
+
+
+def test_counts_identical_over_repeated_runs(labeler):
+    """Two hundred labelings of the same 512^3 Bernoulli p=0.1 volume (1.69 M
+    components, the case with the most concurrent unions) from device memory:
+    every run must report the same, correct component count (a lost union in
+    the global merge would leave one extra root in some runs)."""
+    from tests import hiprt
+
+    entry = _large().get("c3_bern512_p1")
+    if entry is None:
+        pytest.skip("large golden manifest not present")
+    spec, gold = entry["spec"], entry["conn26"]["components"]
+    s = spec["shape"][0]
+    vol = L.bernoulli(tuple(spec["shape"]), spec["p"], spec["seed"])
+    d_v = hiprt.to_device(vol)
+    d_lab = hiprt.DeviceBuffer(s * s * s * 4)
+    d_n = hiprt.DeviceBuffer(4)
+    del vol
+    print(f"[3d repeated runs p=0.1, golden count {gold}]", flush=True)
+    counts = set()
+    for _ in range(200):
+        labeler.label3d_dev(d_v.addr, s, s, s, d_lab.addr, "bleq", 0, d_n.addr)
+        hiprt.synchronize()
+        counts.add(int(d_n.download((1,), np.uint32)[0]))
+    assert counts == {gold}, counts
