@@ -212,3 +212,29 @@ def test_autotune_then_label(tmp_path):
         assert json.loads(p.read_text()) == table
     finally:
         lab.close()
+
+
+def test_counts_identical_over_repeated_runs(labeler):
+    """One hundred labelings of the 8192^2 Bernoulli p=0.5 golden input (one
+    component spanning the image plus 220 k others: the heaviest union work
+    in the tile and merge kernels) from device memory: every run must report
+    the golden component count."""
+    from tests import hiprt
+
+    entry = _large().get("c2_bern8192_p5")
+    if entry is None:
+        pytest.skip("large golden manifest not present")
+    spec, gold = entry["spec"], entry["conn8"]["components"]
+    h, w = spec["shape"]
+    img = L.bernoulli((h, w), spec["p"], spec["seed"])
+    d_px = hiprt.to_device(img)
+    d_lab = hiprt.DeviceBuffer(h * w * 4)
+    d_n = hiprt.DeviceBuffer(4)
+    del img
+    print(f"[bleq2d repeated runs p=0.5, golden count {gold}]", flush=True)
+    counts = set()
+    for _ in range(100):
+        labeler.label2d_dev(d_px.addr, h, w, w, d_lab.addr, "bleq", 0, d_n.addr)
+        hiprt.synchronize()
+        counts.add(int(d_n.download((1,), np.uint32)[0]))
+    assert counts == {gold}, counts
