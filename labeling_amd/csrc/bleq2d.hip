@@ -51,13 +51,6 @@ constexpr uint32_t NT_FINAL = 256;
 using Forest = PackedForest<4>;
 
 // Block mask bits: 1 = (0,0), 2 = (1,0), 4 = (0,1), 8 = (1,1).
-// Pair tests for a block `c` and an earlier neighbour `n` in direction
-// left, up-left, up, up-right: a foreground pixel of c is 8-adjacent to a
-// foreground pixel of n.
-__device__ __forceinline__ bool adj_left(uint32_t c, uint32_t n) { return (c & 0x5u) && (n & 0xAu); }
-__device__ __forceinline__ bool adj_upleft(uint32_t c, uint32_t n) { return (c & 0x1u) && (n & 0x8u); }
-__device__ __forceinline__ bool adj_up(uint32_t c, uint32_t n) { return (c & 0x3u) && (n & 0xCu); }
-__device__ __forceinline__ bool adj_upright(uint32_t c, uint32_t n) { return (c & 0x2u) && (n & 0x4u); }
 
 // ---- geometry sources ----------------------------------------------------------
 

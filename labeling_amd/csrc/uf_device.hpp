@@ -49,53 +49,6 @@ __device__ __forceinline__ uint32_t lds_min(uint32_t *p, uint32_t v)
     return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// ---- LDS forest: parent of element x stored at par[Map(x)] ------------------
-
-// Identity placement.
-struct LinearMap {
-    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return x; }
-};
-
-// Element x = SEG * t + k (block k of thread t) stored at k * NT + t: when a
-// wave touches block k of each of its threads the 64 words are consecutive,
-// i.e. in distinct LDS banks.  With x in place, the 64 lanes would hit
-// 32 / SEG banks only.
-template <uint32_t SEG, uint32_t NT>
-struct SwizzleMap {
-    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return (x % SEG) * NT + x / SEG; }
-};
-
-// Root of x with path halving: x's entry is lowered to its grandparent (by
-// atomicMin, so an entry never grows) before stepping there.  Runs of a
-// dense tile link into long chains without it.
-template <class Map = LinearMap>
-__device__ __forceinline__ uint32_t lds_find(uint32_t *par, uint32_t x, Map at = Map())
-{
-    uint32_t p = lds_load(&par[at(x)]);
-    while (p != x) {
-        const uint32_t g = lds_load(&par[at(p)]);
-        if (g == p) return p;
-        lds_min(&par[at(x)], g);
-        x = g;
-        p = lds_load(&par[at(x)]);
-    }
-    return x;
-}
-
-template <class Map = LinearMap>
-__device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b, Map at = Map())
-{
-    for (;;) {
-        a = lds_find(par, a, at);
-        b = lds_find(par, b, at);
-        if (a == b) return;
-        if (a < b) { uint32_t t = a; a = b; b = t; }
-        uint32_t old = lds_min(&par[at(a)], b);
-        if (old == a) return;
-        a = old;
-    }
-}
-
 // ---- global forest with a packed low field ---------------------------------
 // word = parent << SHIFT | low, where `low` (the block's foreground mask) never
 // changes.  SHIFT = 0 with an all-zero low field is the plain parent array.
@@ -549,16 +502,6 @@ __device__ __forceinline__ uint32_t nz_bytes(uint32_t v)
 {
     uint32_t t = ((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v;
     return (t >> 7) & 0x01010101u;
-}
-
-// Two 4-pixel words of consecutive rows -> the 2x2 masks of the two blocks
-// they cover, as two nibbles (bit0 = top-left, bit1 = top-right,
-// bit2 = bottom-left, bit3 = bottom-right).
-__device__ __forceinline__ uint32_t quad_masks(uint32_t top, uint32_t bot)
-{
-    uint32_t t = nz_bytes(top), b = nz_bytes(bot);
-    return (t & 1u) | ((t >> 7) & 2u) | ((b & 1u) << 2) | ((b >> 5) & 8u) |
-           ((t >> 12) & 0x10u) | ((t >> 19) & 0x20u) | ((b >> 10) & 0x40u) | ((b >> 17) & 0x80u);
 }
 
 }  // namespace lbl
