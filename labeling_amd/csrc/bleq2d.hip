@@ -178,8 +178,15 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
         const uint32_t y0 = CONN4 ? by : 2 * by, x0 = 2 * bxs;
         const uint8_t *row0 = G.px + (size_t)y0 * G.pitch;
         const bool vec = (G.flags & kVecIn) != 0;
-        a = row_bits32(row0, x0, w, vec);
-        if (!CONN4 && y0 + 1 < h) b = row_bits32(row0 + G.pitch, x0, w, vec);
+        if (CONN4) {
+            a = row_bits32(row0, x0, w, vec);
+        } else {
+            const size_t off[2] = {0, G.pitch};
+            const bool use[2] = {true, y0 + 1 < h};
+            uint32_t m[2];
+            rows_bits32<2>(row0, off, use, x0, w, vec, m);
+            a = m[0]; b = m[1];
+        }
     }
     const uint32_t o = a | b;
     const uint32_t st = o & ~(o << 1);               // first column of each segment-local run
@@ -436,18 +443,21 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
         const uint32_t by0 = ty * TBH, bxs = tx * TBW + j * LANE_BLOCKS;
         if (ty > 0 && bxs < bw) {
             const uint32_t *own = G->P + (size_t)by0 * bw + bxs, *up = own - bw;
+            // the blocks left and right of the upper segment, loaded unconditionally
+            // (the segment's first block where there is none) and ahead of the
+            // segments, so that all the loads are in flight at once
+            const bool hl = !CONN4 && bxs > 0, hr = !CONN4 && bxs + LANE_BLOCKS < bw;
+            uint32_t wl = *(hl ? up - 1 : up), wr = *(hr ? up + LANE_BLOCKS : up);
             uint32_t vo[LANE_BLOCKS], vu[LANE_BLOCKS];
             load_segment(G->P + (size_t)by0 * bw, bxs, bw, vo);
             load_segment(G->P + (size_t)(by0 - 1) * bw, bxs, bw, vu);
+            asm volatile("" : "+v"(wl), "+v"(wr));   // no sinking of those loads into their uses
             uint32_t a, b, au, bu;
             band_rows(vo, a, b);
             band_rows(vu, au, bu);
             const uint32_t o = a | b, ou = au | bu;
-            uint32_t bul = 0, bur = 0;
-            if (!CONN4) {
-                bul = bxs > 0 ? (up[-1] & 8u) << 28 : 0u;                       // bottom-right pixel -> bit 31
-                bur = bxs + LANE_BLOCKS < bw ? (up[LANE_BLOCKS] >> 2) & 1u : 0u;   // bottom-left pixel -> bit 0
-            }
+            const uint32_t bul = hl ? (wl & 8u) << 28 : 0u;   // bottom-right pixel -> bit 31
+            const uint32_t bur = hr ? (wr >> 2) & 1u : 0u;    // bottom-left pixel -> bit 0
             // the band above touches with its bottom row (CONN4: its only row)
             SegContacts cs(a, CONN4 ? au : bu, bul, bur, o & ~(o << 1), ou & ~(ou << 1));
             if (CONN4) cs.el = cs.er = 0;

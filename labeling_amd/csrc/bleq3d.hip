@@ -145,10 +145,11 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         const uint32_t x0 = 2 * bxs, y0 = 2 * by, z0 = 2 * bz;
         const bool y1 = y0 + 1 < h, z1 = z0 + 1 < d;
         const uint8_t *r00 = vox + ((size_t)z0 * h + y0) * w;
-        a = row_bits32(r00, x0, w, VEC);
-        if (y1) b = row_bits32(r00 + w, x0, w, VEC);
-        if (z1) c = row_bits32(r00 + (size_t)h * w, x0, w, VEC);
-        if (z1 && y1) e = row_bits32(r00 + (size_t)h * w + w, x0, w, VEC);
+        const size_t plane = (size_t)h * w, off[4] = {0, w, plane, plane + w};
+        const bool use[4] = {true, y1, z1, z1 && y1};
+        uint32_t m[4];
+        rows_bits32<4>(r00, off, use, x0, w, VEC, m);
+        a = m[0]; b = m[1]; c = m[2]; e = m[3];
     }
     const uint32_t o = a | b | c | e;
     const uint32_t st = o & ~(o << 1);                 // first column of each segment-local run

@@ -345,6 +345,38 @@ __device__ __forceinline__ uint32_t row_bits32(const uint8_t *row, uint32_t x0, 
     return m;
 }
 
+// The same for N rows at byte offsets off[i] from row0 (mask 0 where
+// !use[i]).  On the vector path every row's loads are issued before the first
+// is converted, so a lane waits for one memory round trip instead of N in a
+// row; an unused row re-reads row0 (cached) instead of branching around its
+// load.
+template <uint32_t N>
+__device__ __forceinline__ void rows_bits32(const uint8_t *row0, const size_t (&off)[N], const bool (&use)[N], uint32_t x0,
+                                            uint32_t w, bool vec, uint32_t (&m)[N])
+{
+    if (vec && x0 + 32 <= w) {
+        uint4 lo[N], hi[N];
+#pragma unroll
+        for (uint32_t i = 0; i < N; ++i) {
+            const uint4 *q = reinterpret_cast<const uint4 *>(row0 + (use[i] ? off[i] : 0) + x0);
+            lo[i] = q[0];
+            hi[i] = q[1];
+        }
+        // Needed here, unconditionally: keeps the compiler from sinking a
+        // load into the branch of its `use` test, behind the other rows' waits.
+#pragma unroll
+        for (uint32_t i = 0; i < N; ++i) {
+            asm volatile("" : "+v"(lo[i].x), "+v"(lo[i].y), "+v"(lo[i].z), "+v"(lo[i].w));
+            asm volatile("" : "+v"(hi[i].x), "+v"(hi[i].y), "+v"(hi[i].z), "+v"(hi[i].w));
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < N; ++i) m[i] = use[i] ? nz_bits16(lo[i]) | (nz_bits16(hi[i]) << 16) : 0u;
+        return;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < N; ++i) m[i] = use[i] ? row_bits32(row0 + off[i], x0, w, false) : 0u;
+}
+
 // ---- run forests of bit-mask segments ------------------------------------------
 // The band / bar tile kernels give each lane a 32-pixel segment (16 blocks) of
 // one row of blocks; all lanes of a row are consecutive lanes of one wave.  A
