@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_GBS = 8000.0
 ALGO_BYTES = 5.0
+TUNING_TABLE = os.path.join(ROOT, "labeling_amd", "tuning_mi355x.json")
 METRIC = "Gpixel/s 2D 8192², Gvoxel/s 3D 512³, batched vs per-image; % HBM roofline"
 
 
@@ -72,11 +73,13 @@ def roofline(units, path_us, traffic=None, scope=""):
             "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic, "scope": scope}
 
 
-def traffic_from_profiles(name):
+def traffic_from_profiles(name, variant):
+    """HBM bytes per labeling from the committed PMC summary, for the kernel
+    variant ("tile,flatten") that ran; None when that variant was not profiled."""
     path = os.path.join(ROOT, "profiles", name)
     try:
-        return json.load(open(path)).get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
+        return json.load(open(path))["variants"][variant]["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
         return None
 
 
@@ -108,7 +111,11 @@ def bench_2d(L, torch, lab, a):
         lab._check(lab.lib.lbl_tuning_load(lab.handle, f.name.encode()))
         os.unlink(f.name)
         tuning = lab.tuning()
-    elif not a.no_autotune:
+    elif not a.autotune and not a.no_autotune:
+        # the tuning table shipped for MI355X (tile shape chosen over all six inputs)
+        lab._check(lab.lib.lbl_tuning_load(lab.handle, TUNING_TABLE.encode()))
+        tuning = lab.tuning()
+    elif a.autotune:
         # the library's autotuner: tile shape and merge schedule for this size class,
         # timed on the densest-component input (p=0.5)
         log("[bench] autotune")
@@ -120,12 +127,15 @@ def bench_2d(L, torch, lab, a):
     kavg, per_input = kernel_breakdown(lab, step, names, max(len(inputs), min(a.steps, 30)))
     path_us = sum(kavg.values())
     px = side * side
-    rl = roofline(px, path_us, traffic_from_profiles("bleq2d_8192_traffic.json"),
+    key = f"bleq/single2d/{int(math.floor(math.log2(px)))}"
+    variant = ",".join(str(int(v)) for v in tuning[key][:2]) if tuning and key in tuning else "0,0"
+    rl = roofline(px, path_us, traffic_from_profiles("bleq2d_8192_traffic.json", variant),
                   "bleq 2D labeling pipeline, one image (all kernels), algorithmic 5 B/px")
     rl["path_us"] = round(path_us, 2)
     rl["kernels_us"] = {k: round(v, 2) for k, v in kavg.items()}
     rl["per_input_us"] = {k: round(v, 2) for k, v in per_input.items()}
-    rl["autotuned"] = tuning
+    rl["variant"] = variant
+    rl["tuning"] = tuning
     out = {"value": px * a.steps / (ms / 1e3) / 1e9, "ms_per_step": ms / a.steps, "wall_s": wall,
            "roofline": rl, "inputs": inputs}
 
@@ -365,7 +375,9 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="2D: default launch configuration, no tuning")
+    ap.add_argument("--autotune", action="store_true",
+                    help="2D: run the library's autotuner on the p=0.5 input instead of loading the shipped table")
     ap.add_argument("--variant", default="", help="TILE,FLATTEN: run this bleq 2D kernel variant, no autotune")
     ap.add_argument("--quick", action="store_true", help="headline 2D line only")
     ap.add_argument("--skip", default="", help="comma list of extras to skip: 3d,batch,serve")

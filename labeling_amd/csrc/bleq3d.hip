@@ -318,8 +318,11 @@ struct Grid3 {
     uint32_t bd, bh, bw, tiles_x, tiles_y, ntiles;
 };
 
+// A workgroup takes MERGE_JOBS face lanes and unites their distinct pairs with
+// all MERGE_THREADS threads (unions are chains of dependent loads).
 constexpr uint32_t MERGE_THREADS = 256;
-constexpr uint32_t PAIR_CAP = 16 * MERGE_THREADS;   // pair set slots per workgroup
+constexpr uint32_t MERGE_JOBS = 256;
+constexpr uint32_t PAIR_CAP = 16 * MERGE_JOBS;   // pair set slots per workgroup
 
 // The four voxel rows of 4 blocks (mask bit dx + 2 dy + 4 dz) as 8 bits each:
 // bit 2k + dx of row (dy, dz) for block k.
@@ -364,7 +367,8 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
     set.init();
     __syncthreads();
 
-    const uint64_t gid = (uint64_t)blockIdx.x * MERGE_THREADS + threadIdx.x;
+    const uint64_t gid = (uint64_t)blockIdx.x * MERGE_JOBS + threadIdx.x;
+    const bool job = threadIdx.x < MERGE_JOBS;
     const uint64_t bar_jobs = (uint64_t)g.ntiles * BAR_LANES;
     uint32_t links = 0;
     uint32_t la = ~0u, lb = ~0u;   // the lane's previous pair
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
         if (!set.insert(ra, rb)) links += Forest::unite(P, ra, rb);   // no room: unite here
     };
 
-    if (gid < bar_jobs) {
+    if (job && gid < bar_jobs) {
         const uint32_t tile = (uint32_t)(gid / BAR_LANES), k = (uint32_t)(gid % BAR_LANES);
         const uint32_t bar = k / LPB, j = k % LPB;
         const uint32_t ly = bar < TY ? bar : ((bar - TY) & 1u) * (TY - 1), lz = bar < TY ? 0u : 1u + (bar - TY) / 2;
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
                 }
             }
         }
-    } else {
+    } else if (job) {
         const uint64_t g2 = gid - bar_jobs;
         const uint32_t tile = (uint32_t)(g2 / X_LANES), k = (uint32_t)(g2 % X_LANES);
         if (tile < g.ntiles) {
@@ -671,7 +675,7 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
     if (!canonical) canon.nwords = 0;   // keys only
     const bool vec_in = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
     const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
-    const uint32_t merge_grid = (uint32_t)(((uint64_t)g.ntiles * (BAR_LANES + X_LANES) + MERGE_THREADS - 1) / MERGE_THREADS);
+    const uint32_t merge_grid = (uint32_t)(((uint64_t)g.ntiles * (BAR_LANES + X_LANES) + MERGE_JOBS - 1) / MERGE_JOBS);
     const uint32_t xgroups = (bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
 
     if (kt) kt->begin(s);
