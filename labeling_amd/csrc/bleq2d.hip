@@ -364,16 +364,25 @@ __device__ __forceinline__ bool border_job(const Src &src, uint32_t gid, uint32_
 // images every border segment of a tile yields the same pair); a batch, whose
 // pairs also carry an image, in a queue.
 
-// A workgroup takes MERGE_JOBS border lanes but unites their pairs with all
-// MERGE_THREADS threads: the top border of a tile in a dense image yields many
-// distinct pairs, and each union is a chain of dependent loads, so spreading
-// them over 8 threads per border lane shortens the unions done in a row by one
-// thread (8192^2, 256x32-block tiles: merge 28.8 -> 23.3 us; the 3D merge,
-// whose workgroups dedupe pairs across more bars, does not gain from this).
+// A single image's workgroup takes MERGE_JOBS border lanes but unites their
+// pairs with all MERGE_THREADS threads: the top border of a tile in a dense
+// image yields many distinct pairs, and each union is a chain of dependent
+// loads, so spreading them over 8 threads per border lane shortens the unions
+// done in a row by one thread (8192^2, 256x32-block tiles: merge 28.8 -> 23.3
+// us; the 3D merge, whose workgroups dedupe pairs across more bars, does not
+// gain from this).  Batches keep one thread per lane: their many small tiles
+// give short queues, and 8x the threads cost more than they unite (4096
+// images: 310 -> 268 Gpixel/s).
 constexpr uint32_t MERGE_THREADS = 256;
 constexpr uint32_t MERGE_JOBS = 32;
 constexpr uint32_t QUEUE_CAP = 8 * MERGE_THREADS;   // pairs per workgroup (batches)
 constexpr uint32_t HASH_CAP = 16 * MERGE_JOBS;      // pair set slots (single image)
+
+template <class Src>
+constexpr uint32_t merge_jobs()
+{
+    return Src::kSingle ? MERGE_JOBS : MERGE_THREADS;
+}
 
 // Block words of a 16-block segment starting at block bxs of a row (zero past bw).
 __device__ __forceinline__ void load_segment(const uint32_t *row, uint32_t bxs, uint32_t bw, uint32_t v[LANE_BLOCKS])
@@ -421,8 +430,9 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
     else if (threadIdx.x == 0) queue.n = 0;
     __syncthreads();
 
-    const uint32_t gid = blockIdx.x * MERGE_JOBS + threadIdx.x, n_top = total_tiles * LPB;
-    const bool job = threadIdx.x < MERGE_JOBS;
+    constexpr uint32_t JOBS = merge_jobs<Src>();
+    const uint32_t gid = blockIdx.x * JOBS + threadIdx.x, n_top = total_tiles * LPB;
+    const bool job = threadIdx.x < JOBS;
     const Geom2D *G = nullptr;
     uint32_t img = 0, links = 0, la = ~0u, lb = ~0u;
     const auto push = [&](uint32_t ra, uint32_t rb) {
@@ -808,7 +818,7 @@ struct Variant {
     {
         const bool canon = canon_chunks != 0;   // some image asks for canonical labels
         const uint32_t merge_grid =
-            (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + MERGE_JOBS - 1) / MERGE_JOBS);
+            (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + merge_jobs<Src>() - 1) / merge_jobs<Src>());
         const uint32_t border_grid = (uint32_t)(((uint64_t)total_tiles * Border<TBW, TBH>::JOBS + 255) / 256);
         hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src, CONN4>), dim3(total_tiles), dim3(THREADS), 0, s, src);
         if (kt) kt->mark(s, "bleq2d_tile");
