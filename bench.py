@@ -390,8 +390,9 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB: only run a plain device copy (counter scale)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (images are labelled on one "
-              f"device; DESIGN.md describes how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (every image is labelled on "
+              f"one device: batches would shard by image, a single image by stripes with one seam exchange)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch   # initialise torch's HIP runtime before the library binds to it

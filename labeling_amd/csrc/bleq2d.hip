@@ -891,7 +891,6 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
     g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
     g.flags &= kRefLabels | kKeys | kPixelKeys | kCanonLabels;
     if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
-    if (g.w % 8 == 0 && reinterpret_cast<uintptr_t>(g.labels) % 16 == 0) g.flags |= kVecOut;
 }
 
 size_t bleq2d_workspace(uint32_t h, uint32_t w, bool canonical)

@@ -8,7 +8,6 @@ namespace lbl {
 
 enum : uint32_t {
     kVecIn = 1u,       // image rows start 16-byte aligned (pitch and base)
-    kVecOut = 2u,      // label rows start 16-byte aligned (w % 8 == 0, base)
     kRefLabels = 4u,   // write the reference's raw label values
     kKeys = 8u,        // keep first-pixel keys of roots (canonical or label equivalence values)
     kPixelKeys = 16u,  // keys are pixel indices (else canonical bit positions)
