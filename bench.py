@@ -6,7 +6,10 @@ binary images -- the six inputs of the single-image configuration (i.i.d.
 Bernoulli p in {0.1, 0.3, 0.5, 0.7, 0.9} and Gaussian-smoothed noise blobs,
 sigma 4 px, 50% quantile), 8-connectivity, uint32 labels.  One step = one
 labeling of one image (inputs cycle), inputs and outputs resident in HBM, no
-host synchronisation inside the timed region.  value = Gpixel/s.
+host synchronisation inside the timed region.  value = Gpixel/s.  The 2D
+launch configuration (tile shape, flatten pass) is the one of the tuning table
+shipped for MI355X, labeling_amd/tuning_mi355x.json (--autotune: run the
+library's autotuner instead; --variant: pin one).
 
 `roofline` covers the labeling pipeline of one image (all its kernels):
 algorithmic 5 B per pixel (1 B in + 4 B label out) over the summed device time
