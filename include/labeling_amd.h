@@ -58,9 +58,20 @@ typedef enum {
      * bleq: root = min 2x2 (2x2x2) block index, block grid ceil(h/2) x ceil(w/2);
      * lbeq: root = min pixel (voxel) index.  Fastest mode; default. */
     LBL_LABELS_ROOT = 0,
-    /* 2D bleq only: the reference OpenCL output values, 1 + min block index in
-     * the block grid of the image zero-padded to ((n>>5)<<5)+32
-     * (LabelingAlgs.cl:202,317; LabelingTools.cpp:103). */
+    /* The reference OpenCL output values, computed from the roots above, with
+     * the reference's padding ((n>>5)<<5)+32 in 2D, LabelingTools.cpp:103, and
+     * (((n+4)>>5)<<5)+32 with the data at offset +2 in 3D, :160-176,187-195):
+     *  2D bleq: 1 + min block index in the padded block grid
+     *           (LabelingAlgs.cl:202,317);
+     *  2D lbeq: the min pixel index in the padded image, 0-based, and pixel
+     *           (0,0) is never labelled: it keeps 0 (background) and neighbours
+     *           do not connect through it (DistrInitKernel / GetLabel,
+     *           LabelingAlgs.cl:31-52);
+     *  3D bleq: 1 + min block index in the padded [d][h][w] block grid
+     *           (BLEQ3D_Init, LabelingAlgs.cl:828);
+     *  3D lbeq: 1 + min voxel index in the padded volume (LabelingAlgs.cl:641).
+     * The output stays dense and cropped to the input's shape; the component
+     * count is the number of distinct nonzero labels written. */
     LBL_LABELS_REFERENCE = 1,
     /* 1..N in raster order of each component's first pixel (voxel), x
      * fastest: the canonical form used for parity, computed on the device.
@@ -69,7 +80,15 @@ typedef enum {
      * each request on its own).  Costs a
      * key pass over the block words plus a bit-array scan (bleq), or a relabel
      * pass over the labels (lbeq), and a larger workspace. */
-    LBL_LABELS_CANONICAL = 2
+    LBL_LABELS_CANONICAL = 2,
+    /* 3D only (lbl_label3d, lbl_label3d_dev): the LBL_LABELS_REFERENCE values
+     * written into the reference's uncropped output volume
+     * (IOCLLabeling3D::Label returns the padded volume, LabelingTools.cpp:196-220):
+     * P3(d) x P3(h) x P3(w) labels, P3(n) = (((n+4)>>5)<<5)+32, the input's
+     * labels at offset +2 in every dimension and 0 elsewhere.  The label buffer
+     * must hold P3(d)*P3(h)*P3(w) elements.  2D calls return LBL_EUNSUPPORTED
+     * (the reference crops 2D output, LabelingTools.cpp:129). */
+    LBL_LABELS_REFERENCE_UNCROPPED = 3
 } lbl_label_mode;
 
 /* Connectivity argument: 0 = default (bleq 2D: 8, lbeq 2D: 4, 3D: 26),
@@ -94,9 +113,13 @@ const char *lbl_status_string(lbl_status s);
 
 lbl_status lbl_set_label_mode(lbl_ctx *ctx, lbl_label_mode mode);
 
-/* Debug knob: cap on the number of tile-local label-equivalence sweeps per
- * tile, for the one path that sweeps (lbeq 2D, 4-connectivity); exceeding it
- * returns LBL_EITER.  0 = no cap (default). */
+/* Debug knob for 2D lbeq.  cap > 0 selects the Kalentev sweep kernels (per-tile
+ * scan/analyze passes to a fixed point in LDS, then border unions; the
+ * reference's DistrScanKernel / DistrAnalizeKernel, LabelingAlgs.cl:86-130)
+ * instead of the run-based block forests, and caps the sweeps per tile;
+ * exceeding the cap returns LBL_EITER from host-buffer calls.  0 = no cap and
+ * the faster block-forest path (default).  LBL_LABELS_REFERENCE is not
+ * available on the sweep path (LBL_EUNSUPPORTED). */
 lbl_status lbl_set_iteration_cap(lbl_ctx *ctx, uint32_t cap);
 
 /* Registry lookup by the reference's algorithm names, mirroring
@@ -248,6 +271,12 @@ lbl_status lbl_server_submit(lbl_server *srv, const lbl_request *req, uint64_t *
 lbl_status lbl_server_poll(lbl_server *srv, lbl_completion *out, uint32_t cap, uint32_t *n);
 /* Blocks until every submitted request has finished. */
 lbl_status lbl_server_drain(lbl_server *srv);
+/* Testing knob: the next n groups (a fused launch or a single request) the
+ * server launches fail at their kernel launch, before anything is enqueued
+ * there, as a failing hipLaunchKernel would.  Their requests complete with
+ * status LBL_EDEVICE and their label buffers are not written; later groups run
+ * normally. */
+lbl_status lbl_server_inject_faults(lbl_server *srv, uint32_t n);
 /* Seconds on the server's clock (the clock of lbl_completion). */
 double lbl_server_now(const lbl_server *srv);
 void lbl_server_destroy(lbl_server *srv);

@@ -30,7 +30,7 @@ STATUS_NAMES = ["LBL_OK", "LBL_EINVAL", "LBL_EUNSUPPORTED", "LBL_ENOMEM", "LBL_E
 BLEQ, LBEQ = 0, 1
 ALGS = {"bleq": BLEQ, "lbeq": LBEQ}
 # lbl_label_mode
-LABELS_ROOT, LABELS_REFERENCE, LABELS_CANONICAL = 0, 1, 2
+LABELS_ROOT, LABELS_REFERENCE, LABELS_CANONICAL, LABELS_REFERENCE_UNCROPPED = 0, 1, 2, 3
 
 _lib = None
 _synth = None
@@ -103,6 +103,7 @@ def _declare(lib):
         "lbl_server_submit": (i32, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
         "lbl_server_poll": (i32, [P, ctypes.POINTER(Completion), u32, pu32]),
         "lbl_server_drain": (i32, [P]),
+        "lbl_server_inject_faults": (i32, [P, u32]),
         "lbl_server_now": (ctypes.c_double, [P]),
         "lbl_server_destroy": (None, [P]),
         "lbl_host_alloc": (P, [sz]),
@@ -261,12 +262,13 @@ class Labeler:
                                          out.ctypes.data, ctypes.byref(n), ctypes.byref(t)))
         return out, int(n.value), float(t.value)
 
-    def label3d(self, vol: np.ndarray, alg="bleq", conn: int = 0) -> Tuple[np.ndarray, int, float]:
+    def label3d(self, vol: np.ndarray, alg="bleq", conn: int = 0, padded: bool = False) -> Tuple[np.ndarray, int, float]:
+        """padded: the output has the reference's padded shape (mode LABELS_REFERENCE_UNCROPPED)."""
         vol = np.ascontiguousarray(vol, dtype=np.uint8)
         if vol.ndim != 3:
             raise ValueError("label3d expects a 3D array")
         d, h, w = vol.shape
-        out = np.empty((d, h, w), dtype=np.uint32)
+        out = np.empty(tuple(ref_pad3(n) for n in (d, h, w)) if padded else (d, h, w), dtype=np.uint32)
         n = ctypes.c_uint32(0)
         t = ctypes.c_float(0)
         self._check(self.lib.lbl_label3d(self._ctx, _alg_id(alg), vol.ctypes.data, d, h, w, conn,
@@ -382,6 +384,10 @@ class Server:
         if st != OK:
             raise LabelingError(st, "server device failure")
 
+    def inject_faults(self, n: int):
+        """Testing knob: the next n launched groups fail at their kernel launch."""
+        self.labeler._check(self.lib.lbl_server_inject_faults(self._s, n))
+
     def now(self) -> float:
         return float(self.lib.lbl_server_now(self._s))
 
@@ -395,6 +401,16 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def ref_pad2(n: int) -> int:
+    """The reference's 2D padded size ((n >> 5) << 5) + 32 (LabelingTools.cpp:103)."""
+    return ((n >> 5) << 5) + 32
+
+
+def ref_pad3(n: int) -> int:
+    """The reference's 3D padded size (((n + 4) >> 5) << 5) + 32, data at +2 (LabelingTools.cpp:160-176)."""
+    return (((n + 4) >> 5) << 5) + 32
 
 
 def batched_workload(n: int, seed: int, lo: int = 64, hi: int = 2048):

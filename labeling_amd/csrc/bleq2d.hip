@@ -187,6 +187,9 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
             rows_bits32<2>(row0, off, use, x0, w, vec, m);
             a = m[0]; b = m[1];
         }
+        // the reference's label equivalence never labels pixel (0,0) and reads
+        // no neighbour through it (DistrInitKernel, GetLabel: LabelingAlgs.cl:37,44-52)
+        if ((G.flags & kDropOrigin) && (by | bxs) == 0) a &= ~1u;
     }
     const uint32_t o = a | b;
     const uint32_t st = o & ~(o << 1);               // first column of each segment-local run
@@ -660,13 +663,21 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
             if (r != prev_root) {
                 prev_root = r;
                 // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
-                if (CONN4)   // the root block (1x2, the component's first) holds its first pixel
+                // label equivalence values: 1 + the first pixel's index, or with
+                // kRefLabels the reference's 0-based index in its padded image
+                // (width 2 ref_bw; DistrInitKernel, LabelingAlgs.cl:37)
+                if (CONN4) {   // the root block (1x2, the component's first) holds its first pixel
+                    const uint32_t y = r / bw, x = 2 * (r % bw) + ((G.P[r] & 1u) ? 0u : 1u);
                     prev_lab = canon ? canon_label(G.canon, r)
-                                     : 1u + (r / bw) * w + 2 * (r % bw) + ((G.P[r] & 1u) ? 0u : 1u);
-                else
-                    prev_lab = canon ? canon_label(G.canon, G.canon.key[r])
-                                     : MODE == kModeFirst ? G.canon.key[r] + 1u
-                                                          : (ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u);
+                                     : ((G.flags & kRefLabels) ? y * (2 * G.ref_bw) + x : 1u + y * w + x);
+                } else if (canon) {
+                    prev_lab = canon_label(G.canon, G.canon.key[r]);
+                } else if (MODE == kModeFirst) {
+                    const uint32_t k = G.canon.key[r];
+                    prev_lab = (G.flags & kRefLabels) ? (k / w) * (2 * G.ref_bw) + k % w : k + 1u;
+                } else {
+                    prev_lab = ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u;
+                }
             }
             lab[k] = prev_lab;
         }
@@ -890,7 +901,7 @@ void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
     g.xgroups = (g.bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
     g.nfinal = g.xgroups * g.bh;
     g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
-    g.flags &= kRefLabels | kKeys | kPixelKeys | kCanonLabels;
+    g.flags &= kRefLabels | kKeys | kPixelKeys | kCanonLabels | kDropOrigin;
     if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
 }
 
@@ -914,7 +925,7 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
     g.count = im.count;
     g.slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
     g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
-    g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
+    g.flags = opt.mode == LBL_LABELS_REFERENCE ? (first ? kRefLabels | kDropOrigin : kRefLabels) : 0u;
     bleq2d_geometry(g, opt.tile2d);
     const bool canon = opt.mode == LBL_LABELS_CANONICAL;
     if (canon || first) {
@@ -955,6 +966,7 @@ hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hip
     g.count = im.count;
     g.slots = static_cast<uint32_t *>(ws);
     g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
+    g.flags = opt.mode == LBL_LABELS_REFERENCE ? kRefLabels | kDropOrigin : 0u;
     bleq2d_geometry(g, 2);   // 128 x 64 blocks, then one pixel row per block row:
     g.bh = g.h;
     g.tiles_x = (g.bw + TBW - 1) / TBW;

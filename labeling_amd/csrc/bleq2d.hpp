@@ -12,6 +12,7 @@ enum : uint32_t {
     kKeys = 8u,        // keep first-pixel keys of roots (canonical or label equivalence values)
     kPixelKeys = 16u,  // keys are pixel indices (else canonical bit positions)
     kCanonLabels = 32u,  // write canonical labels (batches: per image)
+    kDropOrigin = 64u,   // pixel (0,0) reads as background (the reference's 2D label-equivalence values)
 };
 
 // One image as the kernels see it (also the element of a batch's device table).

@@ -26,6 +26,8 @@
 //              labels of the 2 blocks, counts roots.
 //
 // Block mask bit of voxel (dx, dy, dz) in {0,1}^3 is dx + 2 dy + 4 dz.
+#include <type_traits>
+
 #include "lbl_internal.hpp"
 #include "canon.hpp"
 #include "uf_device.hpp"
@@ -510,6 +512,21 @@ enum : uint32_t {
     kModeRoot = 0,    // 1 + root block index
     kModeCanon = 1,   // canonical numbering (keys: bit positions)
     kModeFirst = 2,   // 1 + the component's first voxel index (keys: voxel indices): the lbeq values
+    kModeRef = 3,     // the reference's bleq values: 1 + padded block index of the root
+    kModeRefFirst = 4,   // the reference's lbeq values: 1 + padded voxel index of the first voxel
+};
+
+// The reference's values and output layout (LBL_LABELS_REFERENCE[_UNCROPPED]):
+// its volume is padded to ref_pad3 per dimension with the data at +2, i.e. +1
+// block, and the values are 1 + the padded block index (bleq, BLEQ3D_Init,
+// LabelingAlgs.cl:828) or 1 + the padded voxel index (lbeq, LBEQ3D_InitKernel,
+// LabelingAlgs.cl:641) of the component's minimum, the same block or voxel as
+// ours since padding keeps the raster order.
+struct Out3 {
+    uint32_t off;        // 2: the reference's uncropped volume (data at +2), 0: dense output
+    uint32_t oh, ow;     // output rows per plane and row length (h, w when dense)
+    uint32_t rbh, rbw;   // padded block grid: ref_pad3(h) / 2, ref_pad3(w) / 2
+    uint32_t rvh, rvw;   // padded volume: ref_pad3(h), ref_pad3(w)
 };
 
 template <bool VEC, uint32_t MODE>
@@ -517,7 +534,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
                                                           uint32_t *__restrict__ labels, uint32_t d, uint32_t h,
                                                           uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
                                                           uint32_t xgroups, uint32_t *count, uint32_t *slots,
-                                                          Canon canon)
+                                                          Canon canon, Out3 o)
 {
     const uint32_t row = blockIdx.x / xgroups;           // block row index bz * bh + by
     const uint32_t by = row % bh, bz = row / bh;
@@ -535,9 +552,21 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
             lab[k] = 0;
             if (!mv[k]) continue;
             if (root[k] != prev_root) {
-                prev_root = root[k];
-                prev_lab = MODE == kModeCanon ? canon_label(canon, canon.key[root[k]])
-                                              : (MODE == kModeFirst ? canon.key[root[k]] + 1u : root[k] + 1u);
+                const uint32_t r = prev_root = root[k];
+                if (MODE == kModeCanon) {
+                    prev_lab = canon_label(canon, canon.key[r]);
+                } else if (MODE == kModeFirst) {   // key: first voxel index
+                    prev_lab = canon.key[r] + 1u;
+                } else if (MODE == kModeRefFirst) {
+                    const uint32_t key = canon.key[r], plane = h * w;
+                    const uint32_t z = key / plane, y = (key % plane) / w, x = key % w;
+                    prev_lab = 1u + ((z + 2) * o.rvh + y + 2) * o.rvw + x + 2;
+                } else if (MODE == kModeRef) {
+                    const uint32_t plane = bh * bw, z = r / plane, y = (r % plane) / bw, x = r % bw;
+                    prev_lab = 1u + ((z + 1) * o.rbh + y + 1) * o.rbw + x + 1;
+                } else {
+                    prev_lab = r + 1u;
+                }
             }
             lab[k] = prev_lab;
         }
@@ -554,9 +583,14 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
                 r[2 * k] = ((mv[k] >> bit) & 1u) ? lab[k] : 0u;
                 r[2 * k + 1] = ((mv[k] >> (bit + 1)) & 1u) ? lab[k] : 0u;
             }
-            uint32_t *dst = labels + ((size_t)z * h + y) * w + x0;
+            uint32_t *dst = labels + ((size_t)(z + o.off) * o.oh + y + o.off) * o.ow + x0 + o.off;
             if (VEC && x0 + 2 * FINAL_BLOCKS <= w) {
-                store_nt4(dst, r[0], r[1], r[2], r[3]);
+                if (o.off) {   // 8-byte aligned in the padded volume
+                    store_nt2(dst, r[0], r[1]);
+                    store_nt2(dst + 2, r[2], r[3]);
+                } else {
+                    store_nt4(dst, r[0], r[1], r[2], r[3]);
+                }
             } else {
                 for (uint32_t i = 0; i < 2 * FINAL_BLOCKS && x0 + i < w; ++i) dst[i] = r[i];
             }
@@ -564,6 +598,18 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_final(const uint32_t *__restr
     }
     // the tile and merge kernels have finished: the first workgroup sums the count
     if (blockIdx.x == 0 && count && threadIdx.x < kCountSlots) count_reduce(slots, count);
+}
+
+// Zeros of the reference's uncropped output outside the input's voxels (the
+// final kernel writes those): one workgroup per output row.
+__global__ __launch_bounds__(256) void bleq3d_pad_zero(uint32_t *__restrict__ out, uint32_t d, uint32_t h, uint32_t w,
+                                                       uint32_t ph, uint32_t pw)
+{
+    const uint32_t z = blockIdx.x / ph, y = blockIdx.x % ph;
+    const bool inner = z >= 2 && z < d + 2 && y >= 2 && y < h + 2;
+    uint32_t *row = out + (size_t)blockIdx.x * pw;
+    for (uint32_t x = threadIdx.x; x < pw; x += 256)
+        if (!inner || x < 2 || x >= w + 2) row[x] = 0;
 }
 
 // ---- first-voxel keys; canonical bits -----------------------------------------------
@@ -677,8 +723,12 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
     const bool canonical = opt.mode == LBL_LABELS_CANONICAL, keys = canonical || first;
     Canon canon = keys ? canon_layout(M + align256(nb), (uint64_t)im.d * im.h * bw, nb) : Canon{};
     if (!canonical) canon.nwords = 0;   // keys only
+    const bool ref = opt.mode == LBL_LABELS_REFERENCE || opt.mode == LBL_LABELS_REFERENCE_UNCROPPED;
+    const bool uncropped = opt.mode == LBL_LABELS_REFERENCE_UNCROPPED;
+    const uint32_t pd = (uint32_t)ref_pad3(im.d), ph = (uint32_t)ref_pad3(im.h), pw = (uint32_t)ref_pad3(im.w);
+    const Out3 out{uncropped ? 2u : 0u, uncropped ? ph : im.h, uncropped ? pw : im.w, ph / 2, pw / 2, ph, pw};
     const bool vec_in = (im.w % 16 == 0) && (reinterpret_cast<uintptr_t>(im.vox) % 16 == 0);
-    const bool vec_out = (im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
+    const bool vec_out = (uncropped || im.w % 8 == 0) && (reinterpret_cast<uintptr_t>(im.labels) % 16 == 0);
     const uint32_t merge_grid = (uint32_t)(((uint64_t)g.ntiles * (BAR_LANES + X_LANES) + MERGE_JOBS - 1) / MERGE_JOBS);
     const uint32_t xgroups = (bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
 
@@ -706,13 +756,24 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
         const hipError_t e = canon_scan_launch(canon, s, kt);
         if (e != hipSuccess) return e;
     }
-    const uint32_t mode = canonical ? kModeCanon : (first ? kModeFirst : kModeRoot);
-    auto fin = vec_out ? (mode == kModeCanon ? bleq3d_final<true, kModeCanon>
-                                             : (mode == kModeFirst ? bleq3d_final<true, kModeFirst> : bleq3d_final<true, kModeRoot>))
-                       : (mode == kModeCanon ? bleq3d_final<false, kModeCanon>
-                                             : (mode == kModeFirst ? bleq3d_final<false, kModeFirst> : bleq3d_final<false, kModeRoot>));
+    const uint32_t mode = canonical ? kModeCanon : (first ? (ref ? kModeRefFirst : kModeFirst) : (ref ? kModeRef : kModeRoot));
+    auto pick = [mode](auto vec) {
+        constexpr bool V = decltype(vec)::value;
+        switch (mode) {
+        case kModeCanon: return bleq3d_final<V, kModeCanon>;
+        case kModeFirst: return bleq3d_final<V, kModeFirst>;
+        case kModeRef: return bleq3d_final<V, kModeRef>;
+        case kModeRefFirst: return bleq3d_final<V, kModeRefFirst>;
+        default: return bleq3d_final<V, kModeRoot>;
+        }
+    };
+    auto fin = vec_out ? pick(std::true_type{}) : pick(std::false_type{});
+    if (uncropped) {
+        hipLaunchKernelGGL(bleq3d_pad_zero, dim3(pd * ph), dim3(256), 0, s, im.labels, im.d, im.h, im.w, ph, pw);
+        if (kt) kt->mark(s, "bleq3d_pad_zero");
+    }
     hipLaunchKernelGGL(fin, fgrid, dim3(NT_FINAL), 0, s, P, M, im.labels, im.d, im.h, im.w, bd, bh, bw, xgroups,
-                       im.count, slots, canon);
+                       im.count, slots, canon, out);
     if (kt) kt->mark(s, "bleq3d_final");
     return hipGetLastError();
 }

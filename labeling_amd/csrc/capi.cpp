@@ -103,9 +103,18 @@ lbl_status normalize_conn3d(lbl_ctx *ctx, lbl_alg alg, int conn)
 lbl_status check_mode(lbl_ctx *ctx, lbl_alg alg, bool is3d)
 {
     const lbl_label_mode m = ctx->opt.mode;
-    if (m == LBL_LABELS_REFERENCE && (alg != LBL_BLEQ || is3d))
-        return fail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_REFERENCE is defined for 2D bleq only");
+    if (m == LBL_LABELS_REFERENCE_UNCROPPED && !is3d)
+        return fail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_REFERENCE_UNCROPPED is defined for 3D only (2D output is cropped)");
+    if (m == LBL_LABELS_REFERENCE && !is3d && alg == LBL_LBEQ && ctx->opt.iter_cap)
+        return fail(ctx, LBL_EUNSUPPORTED, "LBL_LABELS_REFERENCE is not available on the lbeq sweep path (iteration cap set)");
     return LBL_OK;
+}
+
+// Elements of a 3D label buffer: the input's, or the reference's padded volume.
+uint64_t labels3d_elems(lbl_label_mode mode, uint32_t d, uint32_t h, uint32_t w)
+{
+    if (mode != LBL_LABELS_REFERENCE_UNCROPPED) return (uint64_t)d * h * w;
+    return (uint64_t)ref_pad3(d) * ref_pad3(h) * ref_pad3(w);
 }
 
 lbl_status check_size2d(lbl_ctx *ctx, lbl_alg alg, uint32_t h, uint32_t w, size_t pitch)
@@ -115,6 +124,9 @@ lbl_status check_size2d(lbl_ctx *ctx, lbl_alg alg, uint32_t h, uint32_t w, size_
     if ((uint64_t)h * w > kMaxElems) return fail(ctx, LBL_EUNSUPPORTED, "image has more than 2^32-2 pixels");
     if (alg == LBL_BLEQ && ((h + 1ull) / 2) * ((w + 1ull) / 2) > kMaxBlocks2D)
         return fail(ctx, LBL_EUNSUPPORTED, "bleq 2D supports at most 2^28 2x2 blocks");
+    // reference values index the reference's padded image (or its block grid)
+    if (ctx->opt.mode == LBL_LABELS_REFERENCE && ref_pad2(h) * ref_pad2(w) > kMaxElems)
+        return fail(ctx, LBL_EUNSUPPORTED, "reference label values need a padded image of at most 2^32-2 pixels");
     return LBL_OK;
 }
 
@@ -122,6 +134,10 @@ lbl_status check_size3d(lbl_ctx *ctx, uint32_t d, uint32_t h, uint32_t w)
 {
     if (d == 0 || h == 0 || w == 0) return fail(ctx, LBL_EINVAL, "input volume is empty");
     if ((uint64_t)d * h * w > kMaxElems) return fail(ctx, LBL_EUNSUPPORTED, "volume has more than 2^32-2 voxels");
+    const lbl_label_mode m = ctx->opt.mode;
+    if ((m == LBL_LABELS_REFERENCE || m == LBL_LABELS_REFERENCE_UNCROPPED) &&
+        labels3d_elems(LBL_LABELS_REFERENCE_UNCROPPED, d, h, w) > kMaxElems)
+        return fail(ctx, LBL_EUNSUPPORTED, "reference label values need a padded volume of at most 2^32-2 voxels");
     return LBL_OK;
 }
 
@@ -192,7 +208,6 @@ extern "C" lbl_status lbl_create(lbl_ctx **out, int device_ordinal)
         lbl_destroy(ctx);
         return st;
     }
-    ctx->opt.status = ctx->d_misc + 1;
     *out = ctx;
     return LBL_OK;
 }
@@ -239,7 +254,8 @@ extern "C" lbl_status lbl_set_label_mode(lbl_ctx *ctx, lbl_label_mode mode)
 {
     if (!ctx) return LBL_EINVAL;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    if (mode != LBL_LABELS_ROOT && mode != LBL_LABELS_REFERENCE && mode != LBL_LABELS_CANONICAL)
+    if (mode != LBL_LABELS_ROOT && mode != LBL_LABELS_REFERENCE && mode != LBL_LABELS_CANONICAL &&
+        mode != LBL_LABELS_REFERENCE_UNCROPPED)
         return fail(ctx, LBL_EINVAL, "unknown label mode");
     ctx->opt.mode = mode;
     return LBL_OK;
@@ -292,7 +308,10 @@ extern "C" int lbl_last_kernel_times(lbl_ctx *ctx, float *us, const char **names
 
 // ---- device-buffer entry points ----------------------------------------------
 
-static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s)
+// status: device word for the kernels' error bits (host-buffer calls, which
+// clear it before and read it after their own launch), or null (device-buffer
+// calls: not reported).
+static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s, uint32_t *status)
 {
     bool fresh = false;
     lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w, ctx->opt.mode), &fresh);
@@ -301,6 +320,7 @@ static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, 
     if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
     KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
     Options opt = ctx->opt;
+    opt.status = status;
     // lbeq with 8-connectivity runs the bleq block pipeline: bleq's tuning applies
     const int talg = (alg == LBL_LBEQ && conn == 8) ? (int)LBL_BLEQ : (int)alg;
     auto it = ctx->tuning.find(TuneKey{talg, 0, size_class_of((uint64_t)im.h * im.w)});
@@ -313,7 +333,7 @@ static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, 
     return LBL_OK;
 }
 
-static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_t s)
+static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_t s, uint32_t *status)
 {
     bool fresh = false;
     lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace3d(alg, im.d, im.h, im.w, ctx->opt.mode), &fresh);
@@ -321,7 +341,9 @@ static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_
     // every workspace starts with the count slots, which runs leave at zero
     if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
     KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
-    hipError_t e = alg == LBL_BLEQ ? bleq3d_run(im, ctx->opt, ctx->ws, s, kt) : lbeq3d_run(im, ctx->opt, ctx->ws, s, kt);
+    Options opt = ctx->opt;
+    opt.status = status;
+    hipError_t e = alg == LBL_BLEQ ? bleq3d_run(im, opt, ctx->ws, s, kt) : lbeq3d_run(im, opt, ctx->ws, s, kt);
     if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq3d launch" : "lbeq3d launch", e);
     return LBL_OK;
 }
@@ -340,7 +362,7 @@ extern "C" lbl_status lbl_label2d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *
     if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
     DeviceGuard g(ctx->device);
     Image2D im{d_px, pitch_bytes, h, w, d_labels, d_ncomp};
-    return run2d(ctx, alg, c, im, pick_stream(ctx, stream));
+    return run2d(ctx, alg, c, im, pick_stream(ctx, stream), nullptr);
 }
 
 extern "C" lbl_status lbl_label3d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_vox, uint32_t d, uint32_t h,
@@ -355,7 +377,7 @@ extern "C" lbl_status lbl_label3d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *
     if ((st = check_mode(ctx, alg, true)) != LBL_OK) return st;
     DeviceGuard g(ctx->device);
     Image3D im{d_vox, d, h, w, d_labels, d_ncomp};
-    return run3d(ctx, alg, im, pick_stream(ctx, stream));
+    return run3d(ctx, alg, im, pick_stream(ctx, stream), nullptr);
 }
 
 // ---- host-buffer entry points --------------------------------------------------
@@ -373,11 +395,7 @@ static lbl_status finish_host(lbl_ctx *ctx, uint32_t *labels, size_t n, uint32_t
         *elapsed_us = ms * 1000.f;
     }
     if (n_components) *n_components = ctx->h_misc[0];
-    if (ctx->h_misc[1] & 1u) {
-        LBL_HIP(ctx, hipMemsetAsync(ctx->d_misc + 1, 0, sizeof(uint32_t), s));
-        LBL_HIP(ctx, hipStreamSynchronize(s));
-        return fail(ctx, LBL_EITER, "iteration cap reached");
-    }
+    if (ctx->h_misc[1] & 1u) return fail(ctx, LBL_EITER, "iteration cap reached");
     return LBL_OK;
 }
 
@@ -403,9 +421,10 @@ extern "C" lbl_status lbl_label2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *px, 
     // the label buffer (4 B/px) stages the rows until the kernels write it
     LBL_HIP(ctx, upload2d(ctx->d_in, dpitch, px, pitch_bytes, w, h, reinterpret_cast<uint8_t *>(ctx->d_lab),
                           n * sizeof(uint32_t), s));
+    LBL_HIP(ctx, hipMemsetAsync(ctx->d_misc + 1, 0, sizeof(uint32_t), s));   // this call's status bits
     LBL_HIP(ctx, hipEventRecord(ctx->ev0, s));
     Image2D im{ctx->d_in, dpitch, h, w, ctx->d_lab, ctx->d_misc};
-    if ((st = run2d(ctx, alg, c, im, s)) != LBL_OK) return st;
+    if ((st = run2d(ctx, alg, c, im, s, ctx->d_misc + 1)) != LBL_OK) return st;
     return finish_host(ctx, labels, n, n_components, elapsed_us);
 }
 
@@ -420,16 +439,17 @@ extern "C" lbl_status lbl_label3d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *vox,
     if ((st = check_size3d(ctx, d, h, w)) != LBL_OK) return st;
     if ((st = check_mode(ctx, alg, true)) != LBL_OK) return st;
     DeviceGuard g(ctx->device);
-    const size_t n = (size_t)d * h * w;
+    const size_t n = (size_t)d * h * w, nl = labels3d_elems(ctx->opt.mode, d, h, w);
     if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_in), &ctx->in_bytes, n, nullptr)) != LBL_OK) return st;
-    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, n * sizeof(uint32_t), nullptr)) != LBL_OK)
+    if ((st = ensure(ctx, reinterpret_cast<void **>(&ctx->d_lab), &ctx->lab_bytes, nl * sizeof(uint32_t), nullptr)) != LBL_OK)
         return st;
     hipStream_t s = ctx->stream;
     LBL_HIP(ctx, hipMemcpyAsync(ctx->d_in, vox, n, hipMemcpyHostToDevice, s));
+    LBL_HIP(ctx, hipMemsetAsync(ctx->d_misc + 1, 0, sizeof(uint32_t), s));   // this call's status bits
     LBL_HIP(ctx, hipEventRecord(ctx->ev0, s));
     Image3D im{ctx->d_in, d, h, w, ctx->d_lab, ctx->d_misc};
-    if ((st = run3d(ctx, alg, im, s)) != LBL_OK) return st;
-    return finish_host(ctx, labels, n, n_components, elapsed_us);
+    if ((st = run3d(ctx, alg, im, s, ctx->d_misc + 1)) != LBL_OK) return st;
+    return finish_host(ctx, labels, nl, n_components, elapsed_us);
 }
 
 extern "C" lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_image2d *images, uint32_t n, int conn,
@@ -457,7 +477,7 @@ extern "C" lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
     for (uint32_t i = 0; i < n; ++i) {
         const lbl_image2d &im = images[i];
         Image2D one{im.px, im.pitch_bytes, im.h, im.w, im.labels, d_ncomp ? d_ncomp + i : nullptr};
-        if ((st = run2d(ctx, alg, c, one, s)) != LBL_OK) return st;   // the workspace is reused in stream order
+        if ((st = run2d(ctx, alg, c, one, s, nullptr)) != LBL_OK) return st;   // the workspace is reused in stream order
     }
     return LBL_OK;
 }

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(NT) void lbeq2d_tile(const uint8_t *__restrict__ px
         }
         if (!__syncthreads_or(changed)) break;
         if (cap && ++iters >= cap) {
-            if (t == 0) atomicOr(status, 1u);
+            if (t == 0 && status) atomicOr(status, 1u);
             break;
         }
 #pragma unroll 4
@@ -338,11 +338,12 @@ hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws,
 {
     // 8-connectivity: the pixels of a 2x2 block are always connected, so the
     // bleq block forest has the same components; its first-pixel key pass gives
-    // the label equivalence values (1 + minimum pixel index)
-    if (conn == 8 && ((im.h + 1ull) / 2) * ((im.w + 1ull) / 2) <= kMaxBlocks2D)
+    // the label equivalence values (1 + minimum pixel index).  The sweeps below
+    // run for the iteration-cap debug knob, and beyond the block forests'
+    // 2^28-block limit.
+    if (conn == 8 && opt.iter_cap == 0 && ((im.h + 1ull) / 2) * ((im.w + 1ull) / 2) <= kMaxBlocks2D)
         return bleq2d_run(im, opt, ws, s, kt, true);
-    // 4-connectivity: 1x2 blocks, whose two pixels are 4-adjacent; the sweeps
-    // below remain for the iteration-cap debug knob and beyond the block limit
+    // 4-connectivity: 1x2 blocks, whose two pixels are 4-adjacent
     if (conn == 4 && opt.iter_cap == 0 && (uint64_t)im.h * ((im.w + 1ull) / 2) <= kMaxBlocks2D)
         return bleq2d_run_conn4(im, opt, ws, s, kt);
     const uint32_t tiles_x = (im.w + TW - 1) / TW, tiles_y = (im.h + TH - 1) / TH;

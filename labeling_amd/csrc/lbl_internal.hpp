@@ -105,6 +105,12 @@ constexpr uint64_t kMaxElems = (1ull << 32) - 2;  // labels are uint32 with 0 = 
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// The reference's padded sizes, which fix its raw label values: 2D
+// ((n >> 5) << 5) + 32 with the image at the origin (LabelingTools.cpp:103);
+// 3D (((n + 4) >> 5) << 5) + 32 with the data at offset +2 (LabelingTools.cpp:160-176,187-195).
+__host__ __device__ inline uint64_t ref_pad2(uint64_t n) { return ((n >> 5) << 5) + 32; }
+__host__ __device__ inline uint64_t ref_pad3(uint64_t n) { return (((n + 4) >> 5) << 5) + 32; }
+
 }  // namespace lbl
 
 // The context object behind the opaque lbl_ctx handle.

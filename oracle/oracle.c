@@ -472,14 +472,19 @@ int orc_lbeq2d(const uint8_t *px, int64_t h, int64_t w, int64_t pitch,
     if (conn != 0 && conn != 4 && conn != 8) return -1;
     int c8 = (conn == 8);                                     /* COH_DEFAULT -> COH_4, LabelingAlgs.cpp:213 */
     setup_threads(threads);
-    int64_t wp = quirks ? w : orc_pad2(w), hp = quirks ? h : orc_pad2(h);
+    if (quirks < 0 || quirks > 2) return -1;
+    /* quirks 1: unpadded (rows wrap); 0 and 2: the GPU padding contract */
+    int64_t wp = quirks == 1 ? w : orc_pad2(w), hp = quirks == 1 ? h : orc_pad2(h);
     uint8_t *img = pad_image2(px, h, w, pitch, hp, wp);
     uint32_t *lab = (uint32_t *)malloc((size_t)(wp * hp) * sizeof(uint32_t));
     if (!img || !lab) { free(img); free(lab); return -2; }
     const int64_t n = wp * hp;
     /* quirks: GetLabel takes a 32-bit unsigned position (LabelingAlgs.cpp:274-277) */
     if (quirks && n > (int64_t)UINT32_MAX) { free(img); free(lab); return -1; }
-    const uint32_t base = quirks ? 0u : 1u;                   /* 0-based (CPU/2D GPU) or 1-based (3D GPU) */
+    /* quirks 1, 2: 0-based labels, so position 0 is never labelled nor read
+     * (GetLabel: pos && pos < maxSize, LabelingAlgs.cpp:274-277, LabelingAlgs.cl:44-52);
+     * quirks 0: 1-based as the reference's 3D variant (LabelingAlgs.cl:641) */
+    const uint32_t base = quirks ? 0u : 1u;
 
     double t0 = now_s();
     /* InitMap, LabelingAlgs.cpp:226-238 / DistrInitKernel, LabelingAlgs.cl:31-38 */

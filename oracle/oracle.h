@@ -64,7 +64,12 @@ int orc_bleq3d(const uint8_t *vox, int64_t d, int64_t h, int64_t w,
  * quirks=1: the reference CPU code as written: no padding (rows wrap), 0-based
  *           labels, and index 0 excluded from neighbour reads
  *           (LabelingAlgs.cpp:236,274-277).  Used to pin the documented
- *           deviations (pixel (0,0) gets label 0). */
+ *           deviations (pixel (0,0) gets label 0).
+ * quirks=2: the reference GPU kernels' raw values (DistrInitKernel, GetLabel,
+ *           DistrScanKernel, LabelingAlgs.cl:31-110): orc_pad2 padding, 0-based
+ *           labels = min padded pixel index, position 0 never labelled nor read
+ *           as a neighbour; output cropped as IOCLLabeling::Label does
+ *           (LabelingTools.cpp:129). */
 int orc_lbeq2d(const uint8_t *px, int64_t h, int64_t w, int64_t pitch,
                int conn, uint32_t *labels, int quirks, int threads,
                uint32_t *iterations, double *seconds);
