@@ -40,19 +40,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def timed_steps(torch, stream, step, n_inputs, warmup, steps):
+def timed_steps(torch, stream, step, n_inputs, warmup, steps, per_step=False):
+    """Time exactly `steps` steps between device synchronisations.  Returns
+    (ms, wall seconds), plus with per_step the device time of every step (ms),
+    from events recorded between the steps on the launch stream."""
     for i in range(warmup):
         step(i)
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1 if per_step else 2)]
     t0 = time.perf_counter()
-    ev0.record(stream)
+    evs[0].record(stream)
     for i in range(steps):
         step(i)
-    ev1.record(stream)
+        if per_step:
+            evs[i + 1].record(stream)
+    evs[-1].record(stream)
     torch.cuda.synchronize()
-    return ev0.elapsed_time(ev1), time.perf_counter() - t0
+    ms = evs[0].elapsed_time(evs[-1])
+    wall = time.perf_counter() - t0
+    if per_step:
+        return ms, wall, [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+    return ms, wall
 
 
 def kernel_breakdown(lab, step, names, reps):
@@ -125,22 +133,30 @@ def bench_2d(L, torch, lab, a):
         lab.autotune2d(d_in[2].data_ptr(), side, side, side, d_lab.data_ptr(), sptr)
         tuning = lab.tuning()
     log(f"[bench] 2D timed {a.steps} steps")
-    ms, wall = timed_steps(torch, stream, step, len(inputs), a.warmup, a.steps)
+    ms, wall, step_ms = timed_steps(torch, stream, step, len(inputs), a.warmup, a.steps, per_step=True)
     names = [n for n, _ in inputs]
+    # every input weighs the same whatever the step count: the mean over inputs
+    # of each input's mean step time (steps cycle the inputs)
+    by_input = {}
+    for i, t in enumerate(step_ms):
+        by_input.setdefault(names[i % len(names)], []).append(t)
+    step_us = 1e3 * sum(sum(v) / len(v) for v in by_input.values()) / len(by_input)
     kavg, per_input = kernel_breakdown(lab, step, names, max(len(inputs), min(a.steps, 30)))
     path_us = sum(kavg.values())
     px = side * side
     key = f"bleq/single2d/{int(math.floor(math.log2(px)))}"
     variant = ",".join(str(int(v)) for v in tuning[key][:2]) if tuning and key in tuning else "0,0"
-    rl = roofline(px, path_us, traffic_from_profiles("bleq2d_8192_traffic.json", variant),
-                  "bleq 2D labeling pipeline, one image (all kernels), algorithmic 5 B/px")
-    rl["path_us"] = round(path_us, 2)
+    # achieved from the same timing as value (the timed steps, launch gaps included)
+    rl = roofline(px, step_us, traffic_from_profiles("bleq2d_8192_traffic.json", variant),
+                  "bleq 2D labeling pipeline, one image (all kernels), algorithmic 5 B/px, over the timed steps")
+    rl["step_us"] = round(step_us, 2)
+    rl["step_us_by_input"] = {k: round(1e3 * sum(v) / len(v), 2) for k, v in by_input.items()}
+    rl["kernels_path_us"] = round(path_us, 2)   # sum of per-kernel event times (profiling pass, no launch gaps)
     rl["kernels_us"] = {k: round(v, 2) for k, v in kavg.items()}
-    rl["per_input_us"] = {k: round(v, 2) for k, v in per_input.items()}
     rl["variant"] = variant
     rl["tuning"] = tuning
-    out = {"value": px * a.steps / (ms / 1e3) / 1e9, "ms_per_step": ms / a.steps, "wall_s": wall,
-           "roofline": rl, "inputs": inputs}
+    out = {"value": px / (step_us * 1e-6) / 1e9, "value_unweighted": px * a.steps / (ms / 1e3) / 1e9,
+           "ms_per_step": ms / a.steps, "wall_s": wall, "roofline": rl, "inputs": inputs}
 
     if not a.quick:
         log("[bench] lbeq 2D")
@@ -188,9 +204,11 @@ def bench_3d(L, torch, lab, a):
         ms, _ = timed_steps(torch, stream, step, len(vols), 3, steps)
         kavg, per_input = kernel_breakdown(lab, step, [n for n, _ in vols], 9)
         path_us = sum(kavg.values())
+        traffic = traffic_from_profiles("bleq3d_512_traffic.json", alg) if s == 512 else None
         r = {"gvox_s": round(s ** 3 * steps / (ms / 1e3) / 1e9, 2), "ms_per_step": round(ms / steps, 4),
-             "roofline": roofline(s ** 3, path_us, None, f"{alg} 3D pipeline, one volume, algorithmic 5 B/voxel")}
-        r["roofline"]["path_us"] = round(path_us, 2)
+             "roofline": roofline(s ** 3, 1e3 * ms / steps, traffic,
+                                  f"{alg} 3D pipeline, one volume, algorithmic 5 B/voxel, over the timed steps")}
+        r["roofline"]["kernels_path_us"] = round(path_us, 2)
         r["roofline"]["kernels_us"] = {k: round(v, 2) for k, v in kavg.items()}
         r["per_input_us"] = {k: round(v, 2) for k, v in per_input.items()}
         if alg == "bleq":
@@ -283,18 +301,8 @@ def bench_serve(L, lab, a):
     pool = []
     log("[bench] serving request pool")
     for i in range(a.serve_pool):
-        u = L.synth_u64(rng_seed, 0x5A, i)
-        blob = (u >> 8) & 1
-        if u % 10 < 8:
-            side = 256 + L.synth_u64(rng_seed, 0x5B, i) % (4096 - 256 + 1)
-            shape = (side, side)
-            img = L.blobs2d(side, side, 4.0, 0.5, seed=i) if blob else \
-                L.bernoulli(shape, 0.05 + 0.9 * (u % 1000) / 1000, rng_seed, stream=i)
-        else:
-            side = 64 + L.synth_u64(rng_seed, 0x5B, i) % (256 - 64 + 1)
-            shape = (side, side, side)
-            img = L.blobs3d(side, side, side, 3.0, 0.4, seed=i) if blob else \
-                L.bernoulli(shape, 0.05 + 0.3 * (u % 1000) / 1000, rng_seed, stream=i)
+        img = L.serving_request(i, rng_seed)
+        shape = img.shape
         pin = L.PinnedArray(shape, np.uint8)
         pin.array[...] = img
         pool.append(pin)
@@ -305,7 +313,7 @@ def bench_serve(L, lab, a):
     log(f"[bench] serving {qps} req/s for {dur} s")
     free_out = list(range(len(outs)))
     ticket_out = {}
-    lat, done_n, dropped, submitted = [], 0, 0, 0
+    lat, done_n, dropped, submitted, failed = [], 0, 0, 0, 0
     arrivals = []
     t = 0.0
     k = 0
@@ -330,6 +338,9 @@ def bench_serve(L, lab, a):
             i += 1
         for tk, st, ncomp, t_sub, t_beg, t_done in srv.poll():
             free_out.append(ticket_out.pop(tk))
+            if st != L.OK:
+                failed += 1   # not a latency sample
+                continue
             lat.append(t_done - t_sub)
             done_n += 1
         time.sleep(0.0002)
@@ -342,7 +353,8 @@ def bench_serve(L, lab, a):
     return {
         "workload": "Poisson arrivals; 80% 2D (side U[256,4096]), 20% 3D (side U[64,256]); half Bernoulli, half "
                     "blobs; host pinned buffers in/out (latency includes PCIe copies); bleq",
-        "qps_target": qps, "seconds": dur, "submitted": submitted, "completed": done_n, "dropped": dropped,
+        "qps_target": qps, "seconds": dur, "submitted": submitted, "completed": done_n, "failed": failed,
+        "dropped": dropped,
         "throughput_req_s": round(done_n / wall, 1),
         "latency_ms": {"p50": q(0.50), "p90": q(0.90), "p99": q(0.99), "max": q(1.0)},
         "streams": a.serve_streams, "budget_mib": a.serve_budget,
@@ -351,21 +363,69 @@ def bench_serve(L, lab, a):
 
 # ---- CPU baseline ----------------------------------------------------------------------------
 
-def cpu_baseline(inputs, seconds_budget):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def _spread(xs):
+    xs = sorted(xs)
+    return {"median": round(xs[len(xs) // 2], 4), "min": round(xs[0], 4), "max": round(xs[-1], 4), "runs": len(xs)}
+
+
+def cpu_baseline(L, inputs, seconds_budget):
+    """The oracle's OpenMP restatements of the reference's CPU labelers, timed
+    on this host: bleq 2D (TLabelEquivalenceX2, LabelingAlgs.cpp:336-529) on the
+    8192^2 inputs with all threads this process may use and with 1 thread; lbeq
+    2D (TLabelDistribution, :211-330) on the 1024^2 p=0.5 inputs; the 3D
+    restatement of the BLEQ3D_* / LBEQ3D_* kernels (no reference CPU 3D labeler
+    exists) on a 512^3 p=0.3 volume.  Timed region = DoLabel (block init,
+    Scan/Analyze loop, write-back; LabelingTools.cpp:27-34).  Gpixel/s per run;
+    median, min and max over runs."""
     from oracle import pyoracle as O
 
-    threads = min(16, os.cpu_count() or 1)
-    total_px, total_s, runs = 0, 0.0, 0
+    host_cpus = os.cpu_count() or 1
+    threads = max(1, min(host_cpus, int(os.environ.get("OMP_NUM_THREADS") or host_cpus)))
     t_start = time.perf_counter()
-    while (time.perf_counter() - t_start < seconds_budget or runs < len(inputs)) and runs < 6 * len(inputs):
-        name, img = inputs[runs % len(inputs)]
+    runs = []
+    k = 0
+    while k < 3 * len(inputs) and (k < len(inputs) or time.perf_counter() - t_start < seconds_budget):
+        name, img = inputs[k % len(inputs)]
         _, info = O.bleq2d(img, threads=threads, with_info=True)
-        total_px += img.size
-        total_s += info["seconds"]
-        runs += 1
-    return {"value": round(total_px / total_s / 1e9, 4), "unit": "Gpixel/s", "cores": threads, "kind": "port",
-            "sample": f"{runs} labelings of the 8192x8192 inputs (cycled), oracle OpenMP restatement of "
-                      f"TLabelEquivalenceX2 on the padded image, DoLabel region only"}
+        runs.append(img.size / info["seconds"] / 1e9)
+        k += 1
+    one = []
+    for name, img in (inputs[2], inputs[-1]):   # p = 0.5 and the blobs, one thread
+        _, info = O.bleq2d(img, threads=1, with_info=True)
+        one.append(img.size / info["seconds"] / 1e9)
+    lbeq = {}
+    for conn in (4, 8):
+        r = []
+        for seed in (0, 1, 2):
+            img = L.bernoulli((1024, 1024), 0.5, seed=seed)
+            for _ in range(3):
+                _, info = O.lbeq2d(img, conn=conn, threads=threads, with_info=True)
+                r.append(img.size / info["seconds"] / 1e9)
+        lbeq[f"lbeq2d_1024_conn{conn}_gpx_s"] = _spread(r)
+    vol = L.bernoulli((512, 512, 512), 0.3, seed=0)
+    vol3 = {}
+    for alg, fn in (("bleq", O.bleq3d), ("lbeq", O.lbeq3d)):
+        _, info = fn(vol, threads=threads, with_info=True)
+        vol3[f"{alg}3d_512_p0.3_gvox_s"] = round(vol.size / info["seconds"] / 1e9, 4)
+    bleq = _spread(runs)
+    return {"value": bleq["median"], "unit": "Gpixel/s", "cores": threads, "kind": "port",
+            "sample": f"{len(runs)} labelings of the six 8192x8192 inputs (cycled) by the oracle's OpenMP "
+                      f"restatement of TLabelEquivalenceX2 on the padded image, DoLabel region only; value = median",
+            "cpu_model": cpu_model(), "host_cpus": host_cpus, "threads": threads,
+            "bleq2d_8192_gpx_s": bleq, "bleq2d_8192_1thread_gpx_s": _spread(one),
+            "lbeq": lbeq, "restatement_3d": vol3, "seconds": round(time.perf_counter() - t_start, 1)}
 
 
 def main():
@@ -431,7 +491,7 @@ def main():
             extra["batched_4096"] = bench_batch(L, torch, lab, a)
         if "serve" not in skip:
             extra["serving"] = bench_serve(L, lab, a)
-    cpu = None if a.no_cpu_baseline else cpu_baseline(r2["inputs"], a.cpu_seconds)
+    cpu = None if a.no_cpu_baseline else cpu_baseline(L, r2["inputs"], a.cpu_seconds)
     line = {
         "metric": METRIC,
         "value": round(r2["value"], 3),
@@ -450,6 +510,9 @@ def main():
                         "Bernoulli p=0.1,0.3,0.5,0.7,0.9 and blobs sigma=4 q=0.5",
             "global_batch": 1,
             "pixels_per_step": a.side * a.side,
+            "weighting": "each of the six inputs weighs the same: value = pixels / mean over inputs of the "
+                         "input's mean step time (events between steps)",
+            "value_unweighted": round(r2["value_unweighted"], 3),
             "wall_s": round(r2["wall_s"], 4),
         },
         "roofline": r2["roofline"],

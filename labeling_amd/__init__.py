@@ -426,6 +426,25 @@ def batched_workload(n: int, seed: int, lo: int = 64, hi: int = 2048):
     return out
 
 
+def serving_request(i: int, seed: int):
+    """Input i of the serving configuration's request stream: 80% 2D squares of
+    side U{256..4096}, 20% 3D cubes of side U{64..256}; half of each are
+    Gaussian-smoothed noise blobs (2D sigma 4, 50% quantile; 3D sigma 3, 40%
+    quantile), half Bernoulli (2D p ~ U[0.05, 0.95], 3D p ~ U[0.05, 0.35]),
+    all from (seed, i)."""
+    u = synth_u64(seed, 0x5A, i)
+    blob = (u >> 8) & 1
+    if u % 10 < 8:
+        side = 256 + synth_u64(seed, 0x5B, i) % (4096 - 256 + 1)
+        if blob:
+            return blobs2d(side, side, 4.0, 0.5, seed=i)
+        return bernoulli((side, side), 0.05 + 0.9 * (u % 1000) / 1000, seed, stream=i)
+    side = 64 + synth_u64(seed, 0x5B, i) % (256 - 64 + 1)
+    if blob:
+        return blobs3d(side, side, side, 3.0, 0.4, seed=i)
+    return bernoulli((side, side, side), 0.05 + 0.3 * (u % 1000) / 1000, seed, stream=i)
+
+
 def header_functions(path: str = HEADER_PATH):
     """Names of the functions declared in a public header (for export checks)."""
     import re

@@ -8,6 +8,9 @@ produced on any machine; each input's SHA-256 is recorded to pin that.
 
   small2d.npz / small3d.npz  small cases in full (inputs + canonical labels)
   large.json                 full-size workloads: input and label hashes, counts
+  batch4096.json             the batched workload (4096 images, H, W ~ U{64..2048},
+                             p ~ U[0.05, 0.95]): per image its shape, component count
+                             and the SHA-256 of its canonical 8-connected labels
 
 Run from the repository root:  python tests/golden/make_goldens.py [--skip-large]
 """
@@ -131,8 +134,9 @@ def large_entries():
                     ["8", "4"]))
     for p in (0.1, 0.3, 0.5, 0.7, 0.9):
         out.append((f"c2_bern8192_p{int(p * 10)}", {"kind": "bernoulli", "shape": [8192, 8192], "p": p, "seed": 0},
-                    ["8"]))
-    out.append(("c2_blob8192", {"kind": "blobs2d", "shape": [8192, 8192], "sigma": 4.0, "q": 0.5, "seed": 0}, ["8"]))
+                    ["8", "4"]))
+    out.append(("c2_blob8192", {"kind": "blobs2d", "shape": [8192, 8192], "sigma": 4.0, "q": 0.5, "seed": 0},
+                ["8", "4"]))
     for p in (0.1, 0.3):
         out.append((f"c3_bern512_p{int(p * 10)}", {"kind": "bernoulli", "shape": [512, 512, 512], "p": p,
                                                    "seed": 0}, ["26"]))
@@ -166,10 +170,44 @@ def write_large():
         json.dump(res, f, indent=1, sort_keys=True)
 
 
+def _batch_image(args):
+    i, h, w, p, seed = args
+    img = L.bernoulli((h, w), p, seed=seed, stream=i)
+    lab, n = label(img, S8)
+    return i, int(n), sha(lab)
+
+
+def write_batch(n=4096, seed=0, workers=8):
+    """The batched workload as the bench and the GPU test build it: image i is
+    L.bernoulli((h, w), p, seed, stream=i) for L.batched_workload(n, seed)[i]."""
+    from multiprocessing import Pool
+
+    specs = L.batched_workload(n, seed)
+    jobs = [(i, h, w, p, seed) for i, (h, w, p) in enumerate(specs)]
+    out = [None] * n
+    with Pool(workers) as pool:
+        for i, count, digest in pool.imap_unordered(_batch_image, jobs, chunksize=8):
+            h, w, _ = specs[i]
+            out[i] = [h, w, count, digest]
+    res = {"spec": {"kind": "batched_workload", "n": n, "seed": seed, "generator": "bernoulli(seed, stream=i)"},
+           "pixels": int(sum(h * w for h, w, _ in specs)),
+           "images": out}
+    with open(os.path.join(OUT, f"batch{n}.json"), "w") as f:
+        json.dump(res, f, separators=(",", ":"))
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-large", action="store_true")
+    ap.add_argument("--only-batch", action="store_true")
+    ap.add_argument("--only-large", action="store_true")
     a = ap.parse_args()
-    write_small()
-    if not a.skip_large:
+    if a.only_batch:
+        write_batch()
+    elif a.only_large:
         write_large()
+    else:
+        write_small()
+        if not a.skip_large:
+            write_large()
+            write_batch()

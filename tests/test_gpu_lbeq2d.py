@@ -94,16 +94,6 @@ def test_serpentine_and_iteration_cap(labeler):
     assert n == 1
 
 
-def test_reference_mode_rejected_for_lbeq(labeler):
-    labeler.set_mode(L.LABELS_REFERENCE)
-    try:
-        with pytest.raises(L.LabelingError) as e:
-            labeler.label2d(np.zeros((4, 4), np.uint8), "lbeq")
-        assert e.value.status == L.EUNSUPPORTED
-    finally:
-        labeler.set_mode(L.LABELS_ROOT)
-
-
 @pytest.mark.slow
 def test_full_size_c1_conn4(labeler):
     entry = json.load(open(os.path.join(GOLD, "large.json")))["c1_bern1024_p5_s0"]
@@ -129,17 +119,76 @@ def test_full_size_conn8(labeler, name):
     assert n == m == entry["conn8"]["components"]
 
 
-@pytest.mark.parametrize("shape", [(257, 511), (1000, 100), (517, 1029)], ids=lambda s: f"{s[0]}x{s[1]}")
-def test_block_path_equals_sweep_path(labeler, shape):
-    """4-connectivity runs on 1x2 blocks; with an iteration cap set, it runs the
-    tile sweeps instead.  Both give the same raw labels."""
-    img = L.bernoulli(shape, 0.5, seed=99)
-    print(f"[lbeq2d block vs sweeps {shape}]", flush=True)
-    fast, n1, _ = labeler.label2d(img, "lbeq", conn=4)
+def _unpadded_values(raw, w):
+    """Oracle labels 1 + min padded pixel index (padded width orc_pad2(w)) as
+    1 + the same pixel's index in the unpadded raster."""
+    wp = L.ref_pad2(w)
+    v = raw.astype(np.int64)
+    out = np.where(v > 0, ((v - 1) // wp) * w + (v - 1) % wp + 1, 0)
+    return out.astype(np.uint32)
+
+
+SWEEP_SHAPES = [(1, 1), (3, 5), (31, 127), (33, 129), (257, 511), (1000, 100), (517, 1029)]
+
+
+@pytest.mark.parametrize("shape", SWEEP_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("p", [0.3, 0.6])
+def test_sweep_path_vs_oracle(labeler, shape, conn, p):
+    """The Kalentev sweep kernels (lbeq2d_tile / merge / flatten / final, run
+    when an iteration cap is set) against the oracle's restatement of
+    TLabelDistribution (LabelingAlgs.cpp:211-330) / DistrScanKernel
+    (LabelingAlgs.cl:86-130) directly: raw values bit-exact, and equal to the
+    block-forest path's."""
+    img = L.bernoulli(shape, p, seed=hash((shape, conn, p)) & 0xFFFF)
+    print(f"[lbeq2d sweeps vs oracle {shape} conn={conn} p={p}]", flush=True)
+    expect = _unpadded_values(O.lbeq2d(img, conn=conn), shape[1])
+    fast, n1, _ = labeler.label2d(img, "lbeq", conn=conn)
     labeler.set_iteration_cap(1 << 30)
     try:
-        swept, n2, _ = labeler.label2d(img, "lbeq", conn=4)
+        swept, n2, _ = labeler.label2d(img, "lbeq", conn=conn)
     finally:
         labeler.set_iteration_cap(0)
-    np.testing.assert_array_equal(fast, swept)
-    assert n1 == n2
+    np.testing.assert_array_equal(swept, expect)
+    np.testing.assert_array_equal(fast, expect)
+    assert n1 == n2 == O.canonical(expect)[1]
+
+
+def test_sweep_path_serpentine_and_small_goldens(labeler):
+    """The worst case for label propagation, and the small scipy goldens, on
+    the sweep path."""
+    ser = np.zeros((301, 999), np.uint8)
+    for r in range(0, 301, 2):
+        ser[r, :] = 255
+        if r + 1 < 301:
+            ser[r + 1, 998 if (r // 2) % 2 == 0 else 0] = 255
+    labeler.set_iteration_cap(1 << 30)
+    try:
+        for conn in (4, 8):
+            lab, n, _ = labeler.label2d(ser, "lbeq", conn=conn)
+            np.testing.assert_array_equal(lab, _unpadded_values(O.lbeq2d(ser, conn=conn), 999))
+            assert n == 1
+        for name, img, l8, l4 in _small2d():
+            for conn, ref in ((4, l4), (8, l8)):
+                lab, n, _ = labeler.label2d(img, "lbeq", conn=conn)
+                np.testing.assert_array_equal(O.canonical(lab)[0], ref, err_msg=f"{name} conn={conn}")
+                assert n == int(ref.max())
+    finally:
+        labeler.set_iteration_cap(0)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", ["c2_bern8192_p1", "c2_bern8192_p5", "c2_bern8192_p9", "c2_blob8192"])
+def test_full_size_conn4_8192(labeler, name):
+    """8192^2 inputs with 4-connectivity (the lbeq default) against scipy goldens."""
+    entry = json.load(open(os.path.join(GOLD, "large.json")))[name]
+    spec = entry["spec"]
+    if spec["kind"] == "bernoulli":
+        img = L.bernoulli(tuple(spec["shape"]), spec["p"], spec["seed"])
+    else:
+        img = L.blobs2d(*spec["shape"], spec["sigma"], spec["q"], spec["seed"])
+    print(f"[lbeq2d conn4 {name}]", flush=True)
+    lab, n, _ = labeler.label2d(img, "lbeq", conn=4)
+    c, m = O.canonical(lab)
+    assert hashlib.sha256(c.tobytes()).hexdigest() == entry["conn4"]["labels_sha256"]
+    assert n == m == entry["conn4"]["components"]

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import labeling_amd as L
+from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +111,84 @@ def test_canonical_mode_mix_equals_alone(labeler):
             assert lab.max() == n   # canonical: labels 1..n
     finally:
         labeler.set_mode(L.LABELS_ROOT)
+
+
+@pytest.mark.timeout(900)
+def test_serving_workload_vs_oracle(labeler):
+    """Requests of the serving configuration itself (2D sides 256..4096, 3D
+    sides 64..256, half blobs, the bench's generator), submitted together to a
+    server with 3 streams so that they overlap: each result equals the
+    oracle's labeling (canonical labels and count, the independent BFS) and
+    the single-request result on the same input."""
+    n = 36
+    srv = L.Server(labeler, streams=3, hbm_budget_bytes=3 << 30, max_batch=8)
+    reqs, tickets = [], {}
+    try:
+        for i in range(n):
+            img = L.serving_request(i, seed=11)
+            alg = "lbeq" if i % 6 == 5 else "bleq"
+            conn = 8 if alg == "lbeq" and img.ndim == 2 else 0
+            pin_in = L.PinnedArray(img.shape, np.uint8)
+            pin_in.array[...] = img
+            pin_out = L.PinnedArray(img.shape, np.uint32)
+            reqs.append((img, alg, conn, pin_in, pin_out))
+        print(f"[serve C5] {n} requests, sides {[r[0].shape[0] for r in reqs]}", flush=True)
+        for k, (img, alg, conn, pin_in, pin_out) in enumerate(reqs):
+            tickets[srv.submit(pin_in.ptr, pin_out.ptr, img.shape, alg, conn)] = k
+        srv.drain()
+        done = srv.poll(cap=n + 8)
+        assert len(done) == n
+        for ticket, status, ncomp, t_sub, t_start, t_done in done:
+            k = tickets[ticket]
+            img, alg, conn, _, pin_out = reqs[k]
+            assert status == L.OK, f"request {k}"
+            lab = pin_out.array
+            expect, m = O.bfs2d(img, conn or 8) if img.ndim == 2 else O.bfs3d(img, 26)
+            np.testing.assert_array_equal(O.canonical(lab)[0], expect, err_msg=f"request {k} {alg} {img.shape}")
+            assert ncomp == m, f"request {k}"
+            if img.ndim == 2:
+                alone, n1, _ = labeler.label2d(img, alg, conn=conn)
+            else:
+                alone, n1, _ = labeler.label3d(img, alg)
+            np.testing.assert_array_equal(lab, alone, err_msg=f"request {k} alone vs mix")
+            assert n1 == ncomp
+    finally:
+        srv.close()
+        for r in reqs:
+            r[3].free()
+            r[4].free()
+
+
+def test_injected_launch_fault_surfaces(labeler):
+    """A group whose kernel launch fails completes with LBL_EDEVICE for each of
+    its requests (labels not written); the server keeps serving."""
+    imgs = [L.bernoulli((200 + 10 * i, 300), 0.5, seed=i) for i in range(6)]
+    srv = L.Server(labeler, streams=1, hbm_budget_bytes=64 << 20, max_batch=3)
+    pins = []
+    try:
+        srv.inject_faults(1)
+        tickets = {}
+        for k, img in enumerate(imgs):
+            pi, po = L.PinnedArray(img.shape, np.uint8), L.PinnedArray(img.shape, np.uint32)
+            pi.array[...] = img
+            po.array[...] = 7
+            pins.append((pi, po))
+            tickets[srv.submit(pi.ptr, po.ptr, img.shape, "bleq")] = k
+        srv.drain()
+        done = sorted(srv.poll(cap=16), key=lambda c: c[0])
+        statuses = [c[1] for c in done]
+        assert statuses[0] == L.EDEVICE
+        assert L.OK in statuses and set(statuses) <= {L.OK, L.EDEVICE}
+        for ticket, status, ncomp, *_ in done:
+            k = tickets[ticket]
+            if status == L.EDEVICE:
+                assert (pins[k][1].array == 7).all()   # untouched
+            else:
+                alone, n, _ = labeler.label2d(imgs[k], "bleq")
+                np.testing.assert_array_equal(pins[k][1].array, alone)
+                assert ncomp == n
+    finally:
+        srv.close()
+        for a, b in pins:
+            a.free()
+            b.free()

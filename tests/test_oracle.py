@@ -231,3 +231,27 @@ def test_canonicalize_matches_numpy():
     for i, v in enumerate(order):
         expect[lab == v] = i + 1
     np.testing.assert_array_equal(a, expect)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("shape,p,origin", [((37, 53), 0.55, 255), ((64, 64), 0.45, 0), ((5, 33), 0.7, 255),
+                                            ((1, 40), 0.8, 255), ((40, 1), 0.8, 255)])
+def test_lbeq2d_gpu_raw_values_closed_form(conn, shape, p, origin):
+    """orc_lbeq2d quirks=2 (the reference GPU kernels' raw values) equals a
+    closed form built from the independent BFS: drop pixel (0,0), label each
+    remaining component by its first pixel's index in the padded image
+    (0-based); pixel (0,0) keeps 0."""
+    img = L.bernoulli(shape, p, seed=sum(shape) + conn)
+    img[0, 0] = origin
+    got = O.lbeq2d(img, conn=conn, quirks=2)
+    dropped = img.copy()
+    dropped[0, 0] = 0
+    lab, n = O.bfs2d(dropped, conn)
+    wp = O.load().orc_pad2(shape[1])
+    ys, xs = np.nonzero(lab)
+    first = np.full(n + 1, np.iinfo(np.int64).max)
+    np.minimum.at(first, lab[ys, xs], ys.astype(np.int64) * wp + xs)
+    expect = np.zeros(shape, np.int64)
+    expect[ys, xs] = first[lab[ys, xs]]
+    np.testing.assert_array_equal(got, expect.astype(np.uint32))
+    assert got[0, 0] == 0

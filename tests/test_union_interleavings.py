@@ -1,0 +1,21 @@
+"""Model check of the concurrent unions on the CPU (tests/cpp/rem_interleave_check.cpp):
+the tile kernels' LDS loop (Rem's splicing, lds_rem_step) and the global
+merges' unite (read-only finds, root link by atomicMin) keep every union under
+random interleavings of single loads and atomicMins, with sequentially
+consistent memory and (for the global unite) with loads that may return stale
+values of their location."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_union_interleavings(tmp_path):
+    exe = str(tmp_path / "rem_check")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", os.path.join(HERE, "cpp", "rem_interleave_check.cpp"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe, "60000"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "shipped variants exact" in r.stdout
