@@ -13,23 +13,7 @@ from tests import hiprt
 pytestmark = pytest.mark.gpu
 
 
-class ImageDesc(ctypes.Structure):
-    _fields_ = [("px", ctypes.c_void_p), ("labels", ctypes.c_void_p), ("h", ctypes.c_uint32),
-                ("w", ctypes.c_uint32), ("pitch_bytes", ctypes.c_size_t)]
-
-
-def _declare(lib):
-    lib.lbl_batch_create.restype = ctypes.c_int
-    lib.lbl_batch_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ImageDesc), ctypes.c_uint32,
-                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_uint, ctypes.POINTER(ctypes.c_void_p)]
-    lib.lbl_batch_run.restype = ctypes.c_int
-    lib.lbl_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-    lib.lbl_batch_config.restype = ctypes.c_int
-    lib.lbl_batch_config.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_uint32)] * 3 + \
-        [ctypes.POINTER(ctypes.c_int)]
-    lib.lbl_batch_destroy.restype = None
-    lib.lbl_batch_destroy.argtypes = [ctypes.c_void_p]
-    return lib
+ImageDesc = L.ImageDesc   # the package's lbl_image2d (its argtypes are declared on load)
 
 
 def _images(n, seed, max_side=300):
@@ -58,7 +42,7 @@ def _upload(imgs, pitch_pad=0):
 
 @pytest.mark.parametrize("flags", [0, 1, 2], ids=["graph", "autotune", "eager"])
 def test_batch_matches_single_image(labeler, flags):
-    lib = _declare(labeler.lib)
+    lib = labeler.lib
     imgs = _images(257, seed=11)
     bufs, descs = _upload(imgs, pitch_pad=flags)   # pitch 16-aligned or not
     d_n = hiprt.DeviceBuffer(4 * len(imgs))
@@ -87,7 +71,7 @@ def test_batch_matches_single_image(labeler, flags):
 
 
 def test_batch_rejects_lbeq_and_bad_images(labeler):
-    lib = _declare(labeler.lib)
+    lib = labeler.lib
     imgs = _images(3, seed=2)
     bufs, descs = _upload(imgs)
     b = ctypes.c_void_p()
@@ -99,7 +83,7 @@ def test_batch_rejects_lbeq_and_bad_images(labeler):
 
 def test_batch_of_mixed_large_images(labeler):
     """A few images of the batched workload's size range (64..2048 per side)."""
-    lib = _declare(labeler.lib)
+    lib = labeler.lib
     sides = [(64, 2048), (2048, 64), (1999, 1031), (517, 733), (2048, 2048), (64, 64), (1000, 1999)]
     imgs = [L.bernoulli(s, 0.05 + 0.15 * k, seed=3, stream=k) for k, s in enumerate(sides)]
     bufs, descs = _upload(imgs)
@@ -124,7 +108,7 @@ def test_batch_canonical_labels(labeler, flags):
     """LBL_LABELS_CANONICAL on a batch: every image's labels are its own
     canonical numbering (1..n in raster order of first pixels), the oracle's,
     on every replay of the captured graph."""
-    lib = _declare(labeler.lib)
+    lib = labeler.lib
     imgs = _images(97, seed=23) + [np.zeros((5, 7), np.uint8), np.full((9, 4), 255, np.uint8)]
     bufs, descs = _upload(imgs, pitch_pad=flags)
     d_n = hiprt.DeviceBuffer(4 * len(imgs))
