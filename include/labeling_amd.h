@@ -164,6 +164,22 @@ lbl_status lbl_label2d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint3
 lbl_status lbl_label3d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_vox, uint32_t d, uint32_t h,
                            uint32_t w, int conn, uint32_t *d_labels, uint32_t *d_ncomp, void *stream);
 
+/* lbl_label2d_dev that allocates nothing.  No counterpart in the reference
+ * (whose labelers create their device buffers per call,
+ * LabelingAlgs.cpp:996-997); it is for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The union-find workspace is a fixed 72 MiB device array
+ * per device, held until the process ends: images whose workspace exceeds it
+ * return LBL_ENOMEM (2D bleq with root or reference values: at most
+ * 18,873,344 2x2 blocks, e.g. 8688 x 8688 pixels; lbeq and canonical values
+ * need more per pixel).  Every call on a device shares that array
+ * and clears what must start at zero on its own stream, so calls that overlap
+ * on one device race: serialise them (one stream, or events).  Labels, count
+ * and errors are those of lbl_label2d_dev. */
+lbl_status lbl_label2d_dev_noalloc(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
+                                   size_t pitch_bytes, int conn, uint32_t *d_labels, uint32_t *d_ncomp,
+                                   void *stream);
+
 /* Device time in microseconds of each kernel of the most recent *_dev call on
  * ctx when profiling is enabled (lbl_set_profiling).  Writes up to `cap`
  * entries to us[] and the kernel names to names[] (static strings), returns

@@ -88,6 +88,7 @@ def _declare(lib):
         "lbl_label3d": (i32, [P, i32, P, u32, u32, u32, i32, P, pu32, pf]),
         "lbl_label2d_dev": (i32, [P, i32, P, u32, u32, sz, i32, P, P, P]),
         "lbl_label3d_dev": (i32, [P, i32, P, u32, u32, u32, i32, P, P, P]),
+        "lbl_label2d_dev_noalloc": (i32, [P, i32, P, u32, u32, sz, i32, P, P, P]),
         "lbl_set_profiling": (i32, [P, i32]),
         "lbl_last_kernel_times": (i32, [P, pf, ctypes.POINTER(ctypes.c_char_p), i32]),
         "lbl_batch_create": (i32, [P, i32, ctypes.POINTER(ImageDesc), u32, i32, P, ctypes.c_uint,

@@ -208,6 +208,12 @@ extern "C" lbl_status lbl_create(lbl_ctx **out, int device_ordinal)
         lbl_destroy(ctx);
         return st;
     }
+    // the device address of this device's fixed scratch (lbl_label2d_dev_noalloc)
+    if ((e = scratch_setup(device_ordinal, &ctx->scratch)) != hipSuccess) {
+        ctx->scratch = nullptr;
+        ctx->scratch_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        (void)hipGetLastError();
+    }
     *out = ctx;
     return LBL_OK;
 }
@@ -311,14 +317,11 @@ extern "C" int lbl_last_kernel_times(lbl_ctx *ctx, float *us, const char **names
 // status: device word for the kernels' error bits (host-buffer calls, which
 // clear it before and read it after their own launch), or null (device-buffer
 // calls: not reported).
-static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s, uint32_t *status)
+// Enqueue the 2D pipeline with workspace ws (at least workspace2d bytes, count
+// slots at zero).
+static lbl_status launch2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s, uint32_t *status,
+                           void *ws, KernelTimer *kt)
 {
-    bool fresh = false;
-    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w, ctx->opt.mode), &fresh);
-    if (st != LBL_OK) return st;
-    // every workspace starts with the count slots, which runs leave at zero
-    if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
-    KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
     Options opt = ctx->opt;
     opt.status = status;
     // lbeq with 8-connectivity runs the bleq block pipeline: bleq's tuning applies
@@ -328,9 +331,46 @@ static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, 
         opt.tile2d = it->second.tile;
         opt.flatten = it->second.flatten;
     }
-    hipError_t e = alg == LBL_BLEQ ? bleq2d_run(im, opt, ctx->ws, s, kt) : lbeq2d_run(im, conn, opt, ctx->ws, s, kt);
+    hipError_t e = alg == LBL_BLEQ ? bleq2d_run(im, opt, ws, s, kt) : lbeq2d_run(im, conn, opt, ws, s, kt);
     if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq2d launch" : "lbeq2d launch", e);
     return LBL_OK;
+}
+
+static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s, uint32_t *status)
+{
+    bool fresh = false;
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w, ctx->opt.mode), &fresh);
+    if (st != LBL_OK) return st;
+    // every workspace starts with the count slots, which runs leave at zero
+    if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
+    return launch2d(ctx, alg, conn, im, s, status, ctx->ws, ctx->profiling ? &ctx->timer : nullptr);
+}
+
+// lbl_label2d_dev's checks, then the pipeline in a caller-provided workspace:
+// nothing is allocated.  `prologue` (enqueued first, on the same stream)
+// prepares the workspace.
+lbl_status lbl::label2d_in_workspace(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
+                                     size_t pitch_bytes, int conn, uint32_t *d_labels, uint32_t *d_ncomp, void *stream,
+                                     void *ws, size_t ws_bytes, hipError_t (*prologue)(void *, hipStream_t))
+{
+    if (!ctx) return LBL_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!d_px || !d_labels) return fail(ctx, LBL_EINVAL, "null buffer");
+    if (!ws) return fail(ctx, LBL_EDEVICE, "no workspace: " + ctx->scratch_error);
+    int c = 0;
+    lbl_status st;
+    if ((st = normalize_conn2d(ctx, alg, conn, &c)) != LBL_OK) return st;
+    if ((st = check_size2d(ctx, alg, h, w, pitch_bytes)) != LBL_OK) return st;
+    if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
+    const size_t need = workspace2d(alg, h, w, ctx->opt.mode);
+    if (need > ws_bytes)
+        return fail(ctx, LBL_ENOMEM, "image needs " + std::to_string(need) + " bytes of workspace, more than the " +
+                                         std::to_string(ws_bytes) + " available");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = pick_stream(ctx, stream);
+    if (prologue) LBL_HIP(ctx, prologue(ws, s));
+    Image2D im{d_px, pitch_bytes, h, w, d_labels, d_ncomp};
+    return launch2d(ctx, alg, c, im, s, nullptr, ws, nullptr);
 }
 
 static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_t s, uint32_t *status)

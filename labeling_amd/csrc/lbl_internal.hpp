@@ -105,6 +105,11 @@ constexpr uint64_t kMaxElems = (1ull << 32) - 2;  // labels are uint32 with 0 = 
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Fixed device scratch of lbl_label2d_dev_noalloc (noalloc.hip): its size, and
+// the device address of the calling device's copy (resolved once per device).
+extern const size_t kScratchBytes;
+hipError_t scratch_setup(int device, void **ptr);
+
 // The reference's padded sizes, which fix its raw label values: 2D
 // ((n >> 5) << 5) + 32 with the image at the origin (LabelingTools.cpp:103);
 // 3D (((n + 4) >> 5) << 5) + 32 with the data at offset +2 (LabelingTools.cpp:160-176,187-195).
@@ -134,4 +139,12 @@ struct lbl_ctx {
     uint32_t *h_misc = nullptr;   // pinned host mirror of d_misc
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::map<lbl::TuneKey, lbl::TuneVal> tuning;
+    void *scratch = nullptr;        // this device's fixed scratch (device address), or null
+    std::string scratch_error;      // why it is null
 };
+
+namespace lbl {
+lbl_status label2d_in_workspace(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
+                                size_t pitch_bytes, int conn, uint32_t *d_labels, uint32_t *d_ncomp, void *stream,
+                                void *ws, size_t ws_bytes, hipError_t (*prologue)(void *, hipStream_t));
+}  // namespace lbl
