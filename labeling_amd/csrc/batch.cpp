@@ -1,11 +1,15 @@
 // batch.cpp -- batches of 2D images as one hipGraph launch, the autotuner and
 // its tuning table.
 //
-// A batch is planned once: every image gets a Geom2D (its tile grid, block-word
-// slice of one shared workspace, first tile / final-workgroup id in the batch)
-// in a device table, and the four batched kernels (tile, merge, flatten, final)
-// are captured into a hipGraph.  lbl_batch_run replays the graph: thousands of
-// images cost four kernel launches instead of four per image.
+// A batch is planned once (batch_plan.hpp, pure host code tested on the CPU):
+// every image gets a Geom2D (its tile grid, block-word slice of one shared
+// workspace, first tile / final-workgroup id in the batch) in a device table,
+// and tables give the image of every tile, final workgroup and scan chunk id.
+// The batched kernels (tile, merge, flatten, final) are captured into a
+// hipGraph; lbl_batch_run replays it: thousands of images cost four kernel
+// launches instead of four per image.  The workspace and the tables are sized
+// for the largest plan of every tile shape the batch will launch (all seven
+// when autotuning), and every upload checks its plan against those sizes.
 //
 // The autotuner times every tile shape (kBleq2dTiles) with and without the
 // flatten pass on the caller's data and keeps the fastest, per workload class
@@ -31,9 +35,14 @@ struct lbl_batch {
     uint32_t n = 0;
     std::vector<lbl_image2d> images;
     uint32_t *d_ncomp = nullptr;
-    std::vector<Geom2D> geoms;
+    std::vector<uint32_t> hs, ws_;              // image heights and widths
+    plan::Batch plan;                          // the plan of the current tile shape
     Geom2D *d_geoms = nullptr;
     void *ws = nullptr;
+    size_t ws_cap = 0;                         // bytes
+    uint32_t *d_tables = nullptr;
+    uint64_t tables_cap = 0;                   // words
+    BatchTables tables{};
     uint32_t total_tiles = 0, total_final = 0, total_chunks = 0;   // chunks: canonical scan (0: off)
     bool canonical = false;
     uint32_t tile = 0;
@@ -92,14 +101,28 @@ struct DevGuard {
     }
 };
 
-// Geometry of every image for one tile shape; returns the workspace bytes.
-size_t plan(lbl_batch *b, uint32_t tile)
+// Plan the batch for one tile shape (host only).
+lbl_status make_plan(lbl_batch *b, uint32_t tile, plan::Batch &out)
 {
-    b->geoms.resize(b->n);
-    uint64_t off = 0;
-    uint32_t tb = 0, fb = 0, cb = 0;
+    if (!plan::plan_batch(b->hs.data(), b->ws_.data(), b->n, tile, b->canonical, out))
+        return bfail(b->ctx, LBL_EUNSUPPORTED, "batch too large: tile or workgroup ids exceed 32 bits");
+    return LBL_OK;
+}
+
+// Make `pl` the current plan: its Geom2D table and id tables go to the device.
+// The workspace and table buffers must already hold the plan.
+lbl_status upload(lbl_batch *b, plan::Batch &pl)
+{
+    lbl_ctx *ctx = b->ctx;
+    if (pl.ws_bytes > b->ws_cap || pl.table_words() > b->tables_cap)
+        return bfail(ctx, LBL_EINVAL, "batch plan exceeds its buffers (" + std::to_string(pl.ws_bytes) + " > " +
+                                          std::to_string(b->ws_cap) + " bytes or " + std::to_string(pl.table_words()) +
+                                          " > " + std::to_string(b->tables_cap) + " table words)");
+    uint8_t *base = static_cast<uint8_t *>(b->ws);
+    std::vector<Geom2D> dev(b->n);
     for (uint32_t i = 0; i < b->n; ++i) {
         const lbl_image2d &im = b->images[i];
+        const plan::Image &pi = pl.images[i];
         Geom2D g{};
         g.px = im.px;
         g.labels = im.labels;
@@ -109,59 +132,40 @@ size_t plan(lbl_batch *b, uint32_t tile)
         g.w = im.w;
         g.flags = b->ctx->opt.mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
         if (b->canonical) g.flags |= kKeys | kCanonLabels;   // bit-position keys
-        bleq2d_geometry(g, tile);
-        g.tile_base = tb;
-        g.final_base = fb;
-        g.chunk_base = cb;
-        g.slots = reinterpret_cast<uint32_t *>(off);   // offsets, rebased once ws exists
-        g.P = reinterpret_cast<uint32_t *>(off + align256(kCountBytes));
-        tb += g.ntiles;
-        fb += g.nfinal;
-        off += align256(kCountBytes) + align256((uint64_t)g.bw * g.bh * sizeof(uint32_t));
-        if (b->canonical) {   // key array, bits, prefix and partial sums of this image
-            const uint64_t nkeys = (uint64_t)g.h * g.bw, nodes = (uint64_t)g.bw * g.bh;
-            g.canon = canon_layout(reinterpret_cast<void *>(off), nkeys, nodes);
-            cb += canon_chunks(g.canon.nwords);
-            off += canon_workspace(nkeys, nodes);
-        }
-        b->geoms[i] = g;
+        bleq2d_geometry(g, pl.tile_id, true);
+        if (g.ntiles != pi.g.ntiles || g.nfinal != pi.g.nfinal || g.bw != pi.g.bw || g.bh != pi.g.bh)
+            return bfail(ctx, LBL_EINVAL, "batch plan and kernel geometry disagree");
+        g.tile_base = pi.tile_base;
+        g.final_base = pi.final_base;
+        g.chunk_base = pi.chunk_base;
+        g.slots = reinterpret_cast<uint32_t *>(base + pi.ws_off);
+        g.P = reinterpret_cast<uint32_t *>(base + pi.ws_off + align256(kCountBytes));
+        if (b->canonical) g.canon = canon_layout(base + pi.canon_off, pi.nkeys, pi.nnodes);
+        dev[i] = g;
     }
-    b->total_tiles = tb;
-    b->total_final = fb;
-    b->total_chunks = cb;
-    b->tile = tile;
-    return (size_t)off;
-}
-
-lbl_status upload(lbl_batch *b)
-{
-    lbl_ctx *ctx = b->ctx;
-    std::vector<Geom2D> dev = b->geoms;
-    uint8_t *base = static_cast<uint8_t *>(b->ws);
-    // every planned pointer is an offset into the workspace (the first image's
-    // count slots are at offset 0, so a null pointer here is offset 0)
-    const auto rebase = [base](auto *&p) {
-        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + reinterpret_cast<uintptr_t>(p));
-    };
-    for (auto &g : dev) {
-        rebase(g.P);
-        rebase(g.slots);
-        if (b->canonical) {
-            rebase(g.canon.key);
-            rebase(g.canon.bits);
-            rebase(g.canon.prefix);
-            rebase(g.canon.partial);
-        }
-    }
+    std::vector<uint32_t> tab;
+    tab.reserve(pl.table_words());
+    tab.insert(tab.end(), pl.tile_img.begin(), pl.tile_img.end());
+    tab.insert(tab.end(), pl.final_img.begin(), pl.final_img.end());
+    tab.insert(tab.end(), pl.chunk_img.begin(), pl.chunk_img.end());
     B_HIP(ctx, hipMemcpyAsync(b->d_geoms, dev.data(), dev.size() * sizeof(Geom2D), hipMemcpyHostToDevice, ctx->stream));
+    if (!tab.empty())
+        B_HIP(ctx, hipMemcpyAsync(b->d_tables, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                  ctx->stream));
     B_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    b->tables = BatchTables{b->d_tables + pl.tile_tab(), b->d_tables + pl.final_tab(), b->d_tables + pl.chunk_tab()};
+    b->total_tiles = pl.total_tiles;
+    b->total_final = pl.total_final;
+    b->total_chunks = pl.total_chunks;
+    b->tile = pl.tile_id;
+    b->plan = std::move(pl);
     return LBL_OK;
 }
 
 lbl_status launch(lbl_batch *b, hipStream_t s)
 {
-    hipError_t e = bleq2d_run_batch(b->d_geoms, b->n, b->total_tiles, b->total_final, b->total_chunks, b->tile,
-                                    b->flatten, s, nullptr);
+    hipError_t e = bleq2d_run_batch(b->d_geoms, b->n, b->tables, b->total_tiles, b->total_final, b->total_chunks,
+                                    b->tile, b->flatten, s, nullptr);
     if (e != hipSuccess) return bhip(b->ctx, "bleq2d batch launch", e);
     return LBL_OK;
 }
@@ -173,8 +177,8 @@ lbl_status build_graph(lbl_batch *b)
     B_HIP(ctx, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t le = bleq2d_run_batch(b->d_geoms, b->n, b->total_tiles, b->total_final, b->total_chunks, b->tile,
-                                         b->flatten, cs, nullptr);
+        hipError_t le = bleq2d_run_batch(b->d_geoms, b->n, b->tables, b->total_tiles, b->total_final,
+                                         b->total_chunks, b->tile, b->flatten, cs, nullptr);
         e = hipStreamEndCapture(cs, &b->graph);
         if (le != hipSuccess) e = le;
     }
@@ -215,14 +219,24 @@ uint64_t mean_pixels(const lbl_batch *b)
     return b->n ? px / b->n : 0;
 }
 
-lbl_status alloc_ws(lbl_batch *b, size_t bytes)
+// Buffers for the largest plan of the given tile shapes: the workspace (zeroed:
+// the count slots start at zero) and the id tables.
+lbl_status alloc_for(lbl_batch *b, const std::vector<uint32_t> &tiles)
 {
-    if (b->ws) {
-        B_HIP(b->ctx, hipFree(b->ws));
-        b->ws = nullptr;
+    size_t ws = 256;
+    uint64_t words = 1;
+    for (uint32_t t : tiles) {
+        plan::Batch pl;
+        lbl_status st = make_plan(b, t, pl);
+        if (st != LBL_OK) return st;
+        ws = std::max<size_t>(ws, pl.ws_bytes);
+        words = std::max<uint64_t>(words, pl.table_words());
     }
-    B_HIP(b->ctx, hipMalloc(&b->ws, std::max<size_t>(bytes, 256)));
-    B_HIP(b->ctx, hipMemset(b->ws, 0, std::max<size_t>(bytes, 256)));   // count slots start at zero
+    B_HIP(b->ctx, hipMalloc(&b->ws, ws));
+    b->ws_cap = ws;
+    B_HIP(b->ctx, hipMemset(b->ws, 0, ws));
+    B_HIP(b->ctx, hipMalloc(reinterpret_cast<void **>(&b->d_tables), words * sizeof(uint32_t)));
+    b->tables_cap = words;
     return LBL_OK;
 }
 
@@ -254,6 +268,10 @@ extern "C" lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
     b->n = n;
     b->canonical = ctx->opt.mode == LBL_LABELS_CANONICAL;
     b->images.assign(images, images + n);
+    for (uint32_t i = 0; i < n; ++i) {
+        b->hs.push_back(images[i].h);
+        b->ws_.push_back(images[i].w);
+    }
     b->d_ncomp = d_ncomp;
     b->use_graph = !(flags & LBL_BATCH_NO_GRAPH);
     auto cleanup = [&](lbl_status st) {
@@ -272,15 +290,17 @@ extern "C" lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
         flatten = it->second.flatten;
     }
     lbl_status st;
-    if (flags & LBL_BATCH_AUTOTUNE) {
-        // workspace for the largest layout, then time every configuration
-        size_t need = 0;
-        for (uint32_t t = 0; t < kBleq2dTileCount; ++t) need = std::max(need, plan(b, t));
-        if ((st = alloc_ws(b, need)) != LBL_OK) return cleanup(st);
+    std::vector<uint32_t> shapes;
+    if (flags & LBL_BATCH_AUTOTUNE)
+        for (uint32_t t = 0; t < kBleq2dTileCount; ++t) shapes.push_back(t);
+    else
+        shapes.push_back(tile);
+    if ((st = alloc_for(b, shapes)) != LBL_OK) return cleanup(st);
+    if (flags & LBL_BATCH_AUTOTUNE) {   // time every configuration on this batch
         float best = -1.f;
-        for (uint32_t t = 0; t < kBleq2dTileCount; ++t) {
-            plan(b, t);
-            if ((st = upload(b)) != LBL_OK) return cleanup(st);
+        for (uint32_t t : shapes) {
+            plan::Batch pl;
+            if ((st = make_plan(b, t, pl)) != LBL_OK || (st = upload(b, pl)) != LBL_OK) return cleanup(st);
             for (int fl = 1; fl >= 0; --fl) {
                 b->flatten = fl != 0;
                 const float us = time_launch(ctx, [&] { return launch(b, ctx->stream); }, 3);
@@ -294,11 +314,11 @@ extern "C" lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
         if (best < 0.f) return cleanup(bfail(ctx, LBL_EDEVICE, "autotune: no configuration ran"));
         ctx->tuning[key] = TuneVal{tile, flatten, best};
     }
-    const size_t need = plan(b, tile);
     b->flatten = flatten;
-    if (!b->ws || !(flags & LBL_BATCH_AUTOTUNE))
-        if ((st = alloc_ws(b, need)) != LBL_OK) return cleanup(st);
-    if ((st = upload(b)) != LBL_OK) return cleanup(st);
+    {
+        plan::Batch pl;
+        if ((st = make_plan(b, tile, pl)) != LBL_OK || (st = upload(b, pl)) != LBL_OK) return cleanup(st);
+    }
     if (b->use_graph && (st = build_graph(b)) != LBL_OK) return cleanup(st);
     *out = b;
     return LBL_OK;
@@ -339,6 +359,7 @@ extern "C" void lbl_batch_destroy(lbl_batch *b)
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
         if (b->graph) (void)hipGraphDestroy(b->graph);
         if (b->d_geoms) (void)hipFree(b->d_geoms);
+        if (b->d_tables) (void)hipFree(b->d_tables);
         if (b->ws) (void)hipFree(b->ws);
     }
     delete b;

@@ -30,9 +30,10 @@
 // count slots (zeroing them for the next run).
 //
 // Every kernel reads its image geometry from a "source": One (a single image
-// passed by value) or Many (a device table of images; a workgroup finds its
-// image by binary search over the images' first tile / workgroup ids), so a
-// batch of thousands of images runs as the same four launches.
+// passed by value) or Many (a device table of images, and the image of every
+// tile / final workgroup / scan chunk id in tables planned on the host,
+// batch_plan.hpp), so a batch of thousands of images runs as the same four
+// launches.
 //
 // HBM traffic per pixel: 1 B image in + 1 B block words out in (1), 1 B block
 // words in + 4 B labels out in (4), plus the border blocks in (2)-(3).
@@ -44,9 +45,10 @@
 namespace lbl {
 namespace {
 
-constexpr uint32_t FINAL_BLOCKS = 2;     // blocks per thread in the final kernel: one 16-byte store per
-                                         // pixel row, so each wave store covers 1 KiB contiguously
-constexpr uint32_t NT_FINAL = 256;
+constexpr uint32_t FINAL_BLOCKS = plan::kFinalBlocksPerThread;   // blocks per thread in the final kernel: one
+                                                                // 16-byte store per pixel row, so each wave
+                                                                // store covers 1 KiB contiguously
+constexpr uint32_t NT_FINAL = plan::kFinalThreads;
 
 using Forest = PackedForest<4>;
 
@@ -79,41 +81,86 @@ struct One {
 struct Many {
     const Geom2D *g;
     uint32_t n;
-    // last image whose first id is <= id
-    template <int FIELD>
-    __device__ __forceinline__ uint32_t find(uint32_t id) const
-    {
-        uint32_t lo = 0, hi = n;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            const uint32_t base = FIELD == 0 ? g[mid].tile_base : (FIELD == 1 ? g[mid].final_base : g[mid].chunk_base);
-            if (base <= id) lo = mid;
-            else hi = mid;
-        }
-        return lo;
-    }
+    BatchTables t;
     static constexpr bool kSingle = false;
-    __device__ __forceinline__ uint32_t image_of_tile(uint32_t gt) const { return find<0>(gt); }
+    __device__ __forceinline__ uint32_t image_of_tile(uint32_t gt) const { return t.tile_img[gt]; }
     __device__ __forceinline__ const Geom2D &image(uint32_t i) const { return g[i]; }
     __device__ __forceinline__ const Geom2D &by_tile(uint32_t gt, uint32_t &lt) const
     {
-        const Geom2D &r = g[find<0>(gt)];
+        const Geom2D &r = g[t.tile_img[gt]];
         lt = gt - r.tile_base;
         return r;
     }
     __device__ __forceinline__ const Geom2D &by_final(uint32_t gw, uint32_t &lw) const
     {
-        const Geom2D &r = g[find<1>(gw)];
+        const Geom2D &r = g[t.final_img[gw]];
         lw = gw - r.final_base;
         return r;
     }
     __device__ __forceinline__ const Geom2D &by_chunk(uint32_t gc, uint32_t &lc) const
     {
-        const Geom2D &r = g[find<2>(gc)];
+        const Geom2D &r = g[t.chunk_img[gc]];
         lc = gc - r.chunk_base;
         return r;
     }
 };
+
+// Final-kernel chunks per workgroup (plan::geometry2d).
+template <class Src>
+constexpr uint32_t final_chunks()
+{
+    return Src::kSingle ? 1u : plan::kBatchFinalChunks;
+}
+
+// The FINAL_BLOCKS consecutive blocks, in raster order, of this thread in chunk
+// `chunk` of its image (chunk c takes blocks c * NT_FINAL * FINAL_BLOCKS on,
+// whatever the rows): the first block k0 lies at (by, bx); n of the blocks lie
+// in the image; they all lie in row by unless bx + FINAL_BLOCKS > bw.
+struct Span {
+    uint32_t k0, by, bx, n;
+};
+
+// n / d for the divisor of g (plan::div_magic; the same arithmetic as plan::div_by)
+__device__ __forceinline__ uint32_t div_by(uint32_t n, const plan::DivMagic &g)
+{
+    const uint32_t t = __umulhi(g.m, n);
+    return (t + ((n - t) >> g.s1)) >> g.s2;
+}
+
+// The chunk's first block row is found once per workgroup (uniform: scalar
+// unit); a lane's offset from there wraps into the next row at most once when
+// rows hold a workgroup's blocks, else by another division.
+__device__ __forceinline__ Span final_span(const Geom2D &G, uint32_t chunk)
+{
+    Span sp;
+    const uint32_t bw = G.bw, k_wg = chunk * (NT_FINAL * FINAL_BLOCKS);
+    const uint32_t by_wg = div_by(k_wg, G.bdiv), off = threadIdx.x * FINAL_BLOCKS;
+    sp.k0 = k_wg + off;
+    const uint32_t nb = bw * G.bh;
+    sp.n = sp.k0 >= nb ? 0u : min(FINAL_BLOCKS, nb - sp.k0);
+    uint32_t by = by_wg, bx = k_wg - by_wg * bw + off;
+    if (bw >= NT_FINAL * FINAL_BLOCKS) {
+        if (bx >= bw) {
+            bx -= bw;
+            ++by;
+        }
+    } else {
+        const uint32_t q = div_by(bx, G.bdiv);
+        by += q;
+        bx -= q * bw;
+    }
+    sp.by = by;
+    sp.bx = bx;
+    return sp;
+}
+
+// Row and column of block k of a span.
+__device__ __forceinline__ void span_block(const Span &sp, uint32_t k, uint32_t bw, uint32_t &by, uint32_t &bx)
+{
+    const bool wrap = sp.bx + k >= bw;
+    by = wrap ? sp.by + 1 : sp.by;
+    bx = wrap ? sp.bx + k - bw : sp.bx + k;
+}
 
 // First-pixel key of block (bx, by) with mask m != 0 (bit 1 << (dx + 2 dy)):
 // its first pixel's index (kPixelKeys), or first-pixel row * bw + bx, the bit
@@ -149,13 +196,114 @@ __device__ __forceinline__ uint32_t block_key(const Geom2D &G, uint32_t by, uint
 
 constexpr uint32_t LANE_BLOCKS = kLaneBlocks;
 
+// dst[k] = v[k] for lo <= k < hi: compile-time indices only (a runtime index
+// into a register array would place it in scratch memory or LDS).
+template <uint32_t N>
+__device__ __forceinline__ void store_words(uint32_t *dst, const uint32_t (&v)[N], uint32_t lo, uint32_t hi)
+{
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k)
+        if (k >= lo && k < hi) dst[k] = v[k];
+}
+
+// The same for four words held in scalars.
+__device__ __forceinline__ void store_words4(uint32_t *dst, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3,
+                                             uint32_t lo, uint32_t hi)
+{
+    if (lo <= 0 && 0 < hi) dst[0] = v0;
+    if (lo <= 1 && 1 < hi) dst[1] = v1;
+    if (lo <= 2 && 2 < hi) dst[2] = v2;
+    if (lo <= 3 && 3 < hi) dst[3] = v3;
+}
+
+// x_o (o < 4) by two selects on the bits of o: a choice between register
+// values that the compiler cannot fold into a runtime index.
+__device__ __forceinline__ uint32_t by_offset(uint32_t o, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3)
+{
+    const uint32_t even = (o & 2u) ? x2 : x0, odd = (o & 2u) ? x3 : x1;
+    return (o & 1u) ? odd : even;
+}
+
+// Word t of the 16-byte-aligned store windows of a lane whose words start o
+// words past an aligned address: its own out[t - o], or for t < o the previous
+// lane's last words (p13, p14, p15 = its out[13..15]).
+__device__ __forceinline__ uint32_t window_word(const uint32_t (&out)[kLaneBlocks], uint32_t p13, uint32_t p14,
+                                                uint32_t p15, uint32_t t, uint32_t o)
+{
+    const auto at = [&](int i) { return i >= 0 ? out[i] : i == -1 ? p15 : i == -2 ? p14 : p13; };
+    const int ti = (int)t;
+    return by_offset(o, at(ti), at(ti - 1), at(ti - 2), at(ti - 3));
+}
+
+// Store a tile lane's LANE_BLOCKS block words at word index `at` of P, the
+// lane's first block being column bxs of its row (inside: the lane holds
+// blocks of the image; min(LANE_BLOCKS, bw - bxs) of its words are), lane j of
+// its band.  ALIGNED: bw % 16 == 0, so every lane inside the image is whole.
+// When the
+// rows are not 16-byte aligned (bw % 4 != 0), each full lane stores the
+// aligned 4-word windows that end inside its own words, the first one
+// completed with the previous lane's last words (by shuffle); a band's first
+// lane starts at its own first word, and a lane whose next lane is not full
+// stores its own last words itself.  Every lane of the wave must call it.
+template <uint32_t LPB, bool ALIGNED>
+__device__ __forceinline__ void store_block_words(uint32_t *P, size_t at, uint32_t bxs, bool inside, uint32_t j,
+                                                  uint32_t bw, const uint32_t (&out)[kLaneBlocks])
+{
+    uint32_t *dst = P + at;
+    const bool full = inside && bxs + kLaneBlocks <= bw;
+    const auto store_whole = [&]() {
+        uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+        for (uint32_t q = 0; q < kLaneBlocks / 4; ++q) d4[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+    };
+    if (ALIGNED) {   // an exact image: every lane inside it is whole
+        if (inside) store_whole();
+        return;
+    }
+    if ((bw % 4u) == 0) {   // aligned rows (uniform per image)
+        if (full) {
+            store_whole();
+            // keeps the compiler from merging the last words' store with the
+            // partial lanes' (one more store instruction per whole lane)
+            asm volatile("" ::: "memory");
+        } else if (inside) {
+            store_words(dst, out, 0, bw - bxs);
+        }
+        return;
+    }
+    const uint32_t p13 = __shfl_up(out[13], 1, 64), p14 = __shfl_up(out[14], 1, 64), p15 = __shfl_up(out[15], 1, 64);
+    if (full) {
+        const uint32_t o = (uint32_t)(at & 3u);
+        uint4 *base = reinterpret_cast<uint4 *>(dst - o);
+        const bool has_prev = j > 0;   // a full or partial lane's previous lane in the band is full
+#pragma unroll
+        for (uint32_t q = 0; q < kLaneBlocks / 4; ++q) {
+            const uint32_t w0 = window_word(out, p13, p14, p15, 4 * q, o);
+            const uint32_t w1 = window_word(out, p13, p14, p15, 4 * q + 1, o);
+            const uint32_t w2 = window_word(out, p13, p14, p15, 4 * q + 2, o);
+            const uint32_t w3 = window_word(out, p13, p14, p15, 4 * q + 3, o);
+            if (q == 0 && !has_prev && o != 0) {   // a band's first lane: its own words only
+                store_words(dst, out, 0, 4 - o);
+            } else {
+                base[q] = make_uint4(w0, w1, w2, w3);
+            }
+        }
+        const bool next_full = j + 1 < LPB && bxs + 2 * kLaneBlocks <= bw;
+        if (o && !next_full) store_words(dst, out, kLaneBlocks - o, kLaneBlocks);
+    } else if (inside) {
+        store_words(dst, out, 0, bw - bxs);
+    }
+}
+
 // CONN4: 4-connectivity (the label equivalence path): blocks are 1x2 pixels and
 // a band is one pixel row; a row run is 4-connected, and only straight
 // contacts join rows.
 // Single images: at most 64 VGPRs (8 waves per SIMD); the kernel needs 65
 // otherwise, which leaves three 512-lane workgroups per CU where LDS holds four
 // (4% faster).  The batch instances would spill under that limit.
-template <uint32_t TBW, uint32_t TBH, class Src, bool CONN4>
+// EXACT: a single image flagged kExact (no row ends, aligned rows): the
+// kernel holds no code for them.
+template <uint32_t TBW, uint32_t TBH, class Src, bool CONN4, bool EXACT>
 __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 128) ? 8 : 1) void bleq2d_tile(Src src)
 {
     constexpr uint32_t LPB = TBW / LANE_BLOCKS;   // lanes per band
@@ -177,14 +325,14 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
     if (by < bh && bxs < bw) {
         const uint32_t y0 = CONN4 ? by : 2 * by, x0 = 2 * bxs;
         const uint8_t *row0 = G.px + (size_t)y0 * G.pitch;
-        const bool vec = (G.flags & kVecIn) != 0;
+        const bool vec = EXACT || (G.flags & kVecIn) != 0;
         if (CONN4) {
-            a = row_bits32(row0, x0, w, vec);
+            a = row_bits32<EXACT>(row0, x0, w, vec);
         } else {
             const size_t off[2] = {0, G.pitch};
             const bool use[2] = {true, y0 + 1 < h};
             uint32_t m[2];
-            rows_bits32<2>(row0, off, use, x0, w, vec, m);
+            rows_bits32<2, EXACT>(row0, off, use, x0, w, vec, m);
             a = m[0]; b = m[1];
         }
         // the reference's label equivalence never labels pixel (0,0) and reads
@@ -270,37 +418,31 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
 
     // block words: root << 4 | mask (background: own index << 4); the tile's
     // components are counted at their roots (the merge subtracts its links)
-    uint32_t roots = 0;
-    if (by < bh && bxs < bw) {
-        uint32_t out[LANE_BLOCKS], cur = 0, rootmask = 0;
+    // every lane computes its words (selects only: no per-block branch; a
+    // lane outside the image has no pixels, so no nodes and no roots, and
+    // stores nothing)
+    const bool inside = by < bh && bxs < bw;
+    uint32_t out[LANE_BLOCKS], cur = 0, rootmask = 0;
 #pragma unroll
-        for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {   // selects only: no per-block branch
-            const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
-            const bool node = (st & (3u << (2 * kk))) != 0;
-            const uint32_t own = g0 + kk;
-            const uint32_t nv = (v[kk] & kResolved) ? v[kk] : own;
-            cur = node ? nv : cur;
-            rootmask |= (node && nv == own) ? 1u << kk : 0u;
-            out[kk] = m ? (cur << 4) | m : own << 4;   // << 4 drops the tag bit
-        }
-        roots = __builtin_popcount(rootmask);
-        // first-pixel keys start at the root block's own first pixel (global
-        // roots are tile-local roots)
-        if (G.flags & kKeys)
-            for (uint32_t rm = rootmask; rm; rm &= rm - 1) {
-                const uint32_t kk = __builtin_ctz(rm);
-                const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
-                G.canon.key[g0 + kk] = block_key(G, by, bxs + kk, m);
-            }
-        uint32_t *dst = G.P + (size_t)by * bw + bxs;
-        if ((bw % 4u) == 0 && bxs + LANE_BLOCKS <= bw) {
-            uint4 *d4 = reinterpret_cast<uint4 *>(dst);
-#pragma unroll
-            for (uint32_t q = 0; q < LANE_BLOCKS / 4; ++q) d4[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
-        } else {
-            for (uint32_t kk = 0; kk < LANE_BLOCKS && bxs + kk < bw; ++kk) dst[kk] = out[kk];
-        }
+    for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {
+        const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
+        const bool node = (st & (3u << (2 * kk))) != 0;
+        const uint32_t own = g0 + kk;
+        const uint32_t nv = (v[kk] & kResolved) ? v[kk] : own;
+        cur = node ? nv : cur;
+        rootmask |= (node && nv == own) ? 1u << kk : 0u;
+        out[kk] = m ? (cur << 4) | m : own << 4;   // << 4 drops the tag bit
     }
+    const uint32_t roots = __builtin_popcount(rootmask);
+    // first-pixel keys start at the root block's own first pixel (global
+    // roots are tile-local roots)
+    if (G.flags & kKeys)
+        for (uint32_t rm = rootmask; rm; rm &= rm - 1) {
+            const uint32_t kk = __builtin_ctz(rm);
+            const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
+            G.canon.key[g0 + kk] = block_key(G, by, bxs + kk, m);
+        }
+    store_block_words<LPB, EXACT>(G.P, (size_t)by * bw + bxs, bxs, inside, j, bw, out);
     if (G.count) count_add(G.slots, roots);   // one image per workgroup: uniform branch
 }
 
@@ -437,7 +579,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
     constexpr uint32_t JOBS = merge_jobs<Src>();
     const uint32_t gid = blockIdx.x * JOBS + threadIdx.x, n_top = total_tiles * LPB;
     const bool job = threadIdx.x < JOBS;
-    const Geom2D *G = nullptr;
+    const Geom2D *G = &src.image(0);   // a valid image for every lane (no null pointer phi)
     uint32_t img = 0, links = 0, la = ~0u, lb = ~0u;
     const auto push = [&](uint32_t ra, uint32_t rb) {
         if (ra == rb || (ra == la && rb == lb)) return;
@@ -453,7 +595,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
                 return;
             }
         }
-        links += Forest::unite(G->P, ra, rb);   // no room: unite here
+        links += Forest::unite_cached(G->P, ra, rb);   // no room: unite here
     };
 
     if (job && gid < n_top) {
@@ -523,7 +665,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
         const uint32_t n = set.size();
         for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
             const uint2 pr = set.pair(i);
-            links += Forest::unite(G0.P, pr.x, pr.y);
+            links += Forest::unite_cached(G0.P, pr.x, pr.y);
         }
         if (G0.count) count_links(G0.slots, links);   // kernel-uniform branch
     } else {
@@ -532,7 +674,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
         for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
             const uint2 e = queue.pair[i];
             const Geom2D &Gi = src.image(queue.img[i]);
-            if (Forest::unite(Gi.P, e.x, e.y) && Gi.count) atomicSub(&Gi.slots[(blockIdx.x % kCountSlots) * kCountStride], 1u);
+            if (Forest::unite_cached(Gi.P, e.x, e.y) && Gi.count) atomicSub(&Gi.slots[(blockIdx.x % kCountSlots) * kCountStride], 1u);
         }
     }
 }
@@ -542,7 +684,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq2d_merge(Src src, uint32_t 
 template <uint32_t TBW, uint32_t TBH, class Src>
 __global__ __launch_bounds__(256) void bleq2d_flatten(Src src, uint32_t total_tiles)
 {
-    const Geom2D *G = nullptr;
+    const Geom2D *G = &src.image(0);   // a valid image for every lane (no null pointer phi)
     uint32_t bx = 0, by = 0, bx0 = 0, by0 = 0, vb = 0;
     const bool valid = border_job<TBW, TBH>(src, blockIdx.x * blockDim.x + threadIdx.x, total_tiles, G, bx, by,
                                             bx0, by0, vb);
@@ -555,7 +697,7 @@ __global__ __launch_bounds__(256) void bleq2d_flatten(Src src, uint32_t total_ti
         const uint32_t vr = dev_load(&P[r]);
         root = vr >> 4;
         if (root != r) {
-            root = Forest::find(P, r);
+            root = Forest::find_cached(P, r);
             dev_min(&P[r], (root << 4) | (vr & 15u));
         }
     }
@@ -568,47 +710,28 @@ __global__ __launch_bounds__(256) void bleq2d_flatten(Src src, uint32_t total_ti
 
 // ---- 4. final kernel -------------------------------------------------------------------
 
-// Four labels of one pixel row starting at column x0 (x0 % 4 == 0): the widest
-// stores the row's actual address alignment allows (16, 8 or 4 bytes), so
-// images of any width keep vector stores on most rows.
-__device__ __forceinline__ void store_row4(uint32_t *dst, const uint32_t v[4], uint32_t x0, uint32_t w)
+// Block words of one lane's FINAL_BLOCKS blocks (zero outside the image).
+__device__ __forceinline__ void load_span(const Geom2D &G, const Span &sp, uint32_t v[FINAL_BLOCKS])
 {
-    const uintptr_t a = reinterpret_cast<uintptr_t>(dst);
-    if (x0 + 4 <= w) {
-        if ((a & 15u) == 0) {
-            store_nt4(dst, v[0], v[1], v[2], v[3]);
-            return;
-        }
-        if ((a & 7u) == 0) {
-            store_nt2(dst, v[0], v[1]);
-            store_nt2(dst + 2, v[2], v[3]);
-            return;
-        }
+    const uint32_t *__restrict__ P = G.P;
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = 0;
+    if (sp.n == FINAL_BLOCKS) {   // k0 even: 8-byte aligned
+        const uint2 q = *reinterpret_cast<const uint2 *>(P + sp.k0);
+        v[0] = q.x; v[1] = q.y;
+    } else if (sp.n) {
+        for (uint32_t k = 0; k < sp.n; ++k) v[k] = P[sp.k0 + k];
     }
-    for (uint32_t i = 0; i < 4 && x0 + i < w; ++i) dst[i] = v[i];
 }
 
-// Block words of one lane's FINAL_BLOCKS blocks (zero outside the image) and
-// the blocks' roots (where the mask is not empty).  Neighbouring lanes mostly
-// share their blocks' parent (a tile-local root): the first lane of each run of
-// lanes with the same first parent chases it, the others take its root.
-__device__ __forceinline__ void lane_blocks(const Geom2D &G, uint32_t by, uint32_t bxs, uint32_t v[FINAL_BLOCKS],
-                                            uint32_t root[FINAL_BLOCKS])
+// Roots of a lane's blocks (where the mask is not empty).  Neighbouring lanes
+// mostly share their blocks' parent (a tile-local root): the first lane of each
+// run of lanes with the same first parent chases it, the others take its root.
+__device__ __forceinline__ void lane_roots(const Geom2D &G, const uint32_t v[FINAL_BLOCKS], uint32_t root[FINAL_BLOCKS])
 {
-    const uint32_t bw = G.bw;
     const uint32_t *__restrict__ P = G.P;
-    const bool inside = by < G.bh && bxs < bw;
 #pragma unroll
-    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = root[k] = 0;
-    if (inside) {
-        const uint32_t *src_w = P + (size_t)by * bw + bxs;
-        if ((reinterpret_cast<uintptr_t>(src_w) & 7u) == 0 && bxs + FINAL_BLOCKS <= bw) {
-            const uint2 q = *reinterpret_cast<const uint2 *>(src_w);
-            v[0] = q.x; v[1] = q.y;
-        } else {
-            for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = bxs + k < bw ? src_w[k] : 0u;
-        }
-    }
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) root[k] = 0;
     uint32_t key = 0;
     bool has = false;
 #pragma unroll
@@ -621,16 +744,14 @@ __device__ __forceinline__ void lane_blocks(const Geom2D &G, uint32_t by, uint32
     const bool lead = run_leader(key, 0u, has, leader);
     uint32_t key_root = lead ? Forest::find_final(P, key) : 0u;
     key_root = __shfl(key_root, (int)leader, 64);
-    if (inside) {
-        uint32_t prev_parent = key, prev_root = key_root;
+    uint32_t prev_parent = key, prev_root = key_root;
 #pragma unroll
-        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
-            if (!(v[k] & 15u)) continue;
-            const uint32_t p = v[k] >> 4;
-            root[k] = p == prev_parent ? prev_root : Forest::find_final(P, p);
-            prev_parent = p;
-            prev_root = root[k];
-        }
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+        if (!(v[k] & 15u)) continue;
+        const uint32_t p = v[k] >> 4;
+        root[k] = p == prev_parent ? prev_root : Forest::find_final(P, p);
+        prev_parent = p;
+        prev_root = root[k];
     }
 }
 
@@ -639,19 +760,53 @@ __device__ __forceinline__ void lane_blocks(const Geom2D &G, uint32_t by, uint32
 // kModeFirst: 1 + the component's first pixel index (the label equivalence values).
 enum : uint32_t { kModeRoot = 0, kModeRef = 1, kModeCanon = 2, kModeFirst = 3, kModeAny = 4 };
 
-template <class Src, uint32_t MODE, bool CONN4>
-__global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
+// Store a lane's four labels v0..v3 of one pixel row, columns x0 .. x0 + 3
+// (those < w), row starting at `row`.  A row whose lanes' labels are not
+// 16-byte aligned (word offset o = address / 4 mod 4 != 0) is written in
+// aligned 4-word windows: each lane stores the window that ends in its own
+// labels, its first o words being the previous lane's last labels (shuffled);
+// a lane without that previous lane (has_prev: the previous lane of the wave
+// holds this row's 4 labels just left of x0) stores its own first 4 - o
+// labels one by one, and a lane whose next lane will not store such a window
+// stores its own last o labels.  write: the lane has labels in this row.
+// Every lane of the wave must call it.
+__device__ __forceinline__ void store_row_aligned(uint32_t *row, uint32_t x0, uint32_t w, uint32_t v0, uint32_t v1,
+                                                  uint32_t v2, uint32_t v3, bool write, bool has_prev, bool next_in_row)
 {
-    uint32_t lw;
-    const Geom2D &G = src.by_final(blockIdx.x, lw);
-    const uint32_t h = G.h, w = G.w, bw = G.bw;
-    const uint32_t by = lw / G.xgroups;
-    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
-    uint32_t v[FINAL_BLOCKS], root[FINAL_BLOCKS];
-    lane_blocks(G, by, bxs, v, root);
+    const uint32_t p1 = __shfl_up(v1, 1, 64), p2 = __shfl_up(v2, 1, 64), p3 = __shfl_up(v3, 1, 64);
+    if (!write) return;
+    uint32_t *dst = row + x0;   // x0 < w
+    const uint32_t o = (uint32_t)((reinterpret_cast<uintptr_t>(dst) >> 2) & 3u), lim = min(4u, w - x0);
+    if (o == 0) {
+        if (lim == 4)
+            store_nt4(dst, v0, v1, v2, v3);
+        else
+            store_words4(dst, v0, v1, v2, v3, 0, lim);
+        return;
+    }
+    // the window: the previous lane's last o labels, then this lane's first 4 - o
+    const uint32_t a0 = by_offset(o, 0, p3, p2, p1);
+    const uint32_t a1 = by_offset(o, 0, v0, p3, p2);
+    const uint32_t a2 = by_offset(o, 0, v1, v0, p3);
+    const uint32_t a3 = by_offset(o, 0, v2, v1, v0);
+    if (has_prev && x0 + 4 - o <= w)
+        store_nt4(dst - o, a0, a1, a2, a3);
+    else
+        store_words4(dst, v0, v1, v2, v3, 0, min(4u - o, lim));
+    if (!(next_in_row && x0 + 8 - o <= w)) store_words4(dst, v0, v1, v2, v3, 4 - o, lim);
+}
 
-    if (by < G.bh && bxs < bw) {
-        uint32_t lab[FINAL_BLOCKS];
+// Labels of one lane's blocks of a span, written out.
+template <uint32_t MODE, bool CONN4, bool EXACT>
+__device__ __forceinline__ void final_span_labels(const Geom2D &G, const Span &sp, const uint32_t v[FINAL_BLOCKS])
+{
+    const uint32_t h = G.h, w = G.w, bw = G.bw;
+    uint32_t root[FINAL_BLOCKS];
+    lane_roots(G, v, root);
+    uint32_t lab[FINAL_BLOCKS];
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) lab[k] = 0;
+    if (sp.n) {
         const bool ref = MODE == kModeRef || (MODE == kModeAny && (G.flags & kRefLabels));
         const bool canon = MODE == kModeCanon || (MODE == kModeAny && (G.flags & kCanonLabels));
         uint32_t prev_root = ~0u, prev_lab = 0;
@@ -662,12 +817,12 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
             const uint32_t r = root[k];
             if (r != prev_root) {
                 prev_root = r;
-                // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317)
+                // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317);
                 // label equivalence values: 1 + the first pixel's index, or with
                 // kRefLabels the reference's 0-based index in its padded image
                 // (width 2 ref_bw; DistrInitKernel, LabelingAlgs.cl:37)
                 if (CONN4) {   // the root block (1x2, the component's first) holds its first pixel
-                    const uint32_t y = r / bw, x = 2 * (r % bw) + ((G.P[r] & 1u) ? 0u : 1u);
+                    const uint32_t y = div_by(r, G.bdiv), x = 2 * (r - y * bw) + ((G.P[r] & 1u) ? 0u : 1u);
                     prev_lab = canon ? canon_label(G.canon, r)
                                      : ((G.flags & kRefLabels) ? y * (2 * G.ref_bw) + x : 1u + y * w + x);
                 } else if (canon) {
@@ -676,27 +831,85 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
                     const uint32_t k = G.canon.key[r];
                     prev_lab = (G.flags & kRefLabels) ? (k / w) * (2 * G.ref_bw) + k % w : k + 1u;
                 } else {
-                    prev_lab = ref ? 1u + (r % bw) + (r / bw) * G.ref_bw : r + 1u;
+                    if (ref) {
+                        const uint32_t ry = div_by(r, G.bdiv);
+                        prev_lab = 1u + (r - ry * bw) + ry * G.ref_bw;
+                    } else {
+                        prev_lab = r + 1u;
+                    }
                 }
             }
             lab[k] = prev_lab;
         }
-        const uint32_t y0 = CONN4 ? by : 2 * by, x0 = 2 * bxs;
-        uint32_t *row0 = G.labels + (size_t)y0 * w + x0;
-        uint32_t *row1 = row0 + w;
-        const bool two_rows = !CONN4 && y0 + 1 < h;
-        uint32_t top[2 * FINAL_BLOCKS], bot[2 * FINAL_BLOCKS];
-#pragma unroll
-        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
-            const uint32_t m = v[k] & 15u;
-            top[2 * k] = (m & 1u) ? lab[k] : 0u;
-            top[2 * k + 1] = (m & 2u) ? lab[k] : 0u;
-            bot[2 * k] = (m & 4u) ? lab[k] : 0u;
-            bot[2 * k + 1] = (m & 8u) ? lab[k] : 0u;
-        }
-        store_row4(row0, top, x0, w);
-        if (two_rows) store_row4(row1, bot, x0, w);
     }
+    // the lane's blocks in one block row: one 16-byte store per pixel row
+    const bool row_path = sp.n && sp.bx + FINAL_BLOCKS <= bw;
+    const uint32_t y0 = CONN4 ? sp.by : 2 * sp.by, x0 = 2 * sp.bx;
+    uint32_t *row0 = G.labels + (size_t)y0 * w;
+    const bool two_rows = !CONN4 && y0 + 1 < h;
+    uint32_t top[2 * FINAL_BLOCKS], bot[2 * FINAL_BLOCKS];
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+        const uint32_t m = v[k] & 15u;
+        top[2 * k] = (m & 1u) ? lab[k] : 0u;
+        top[2 * k + 1] = (m & 2u) ? lab[k] : 0u;
+        bot[2 * k] = (m & 4u) ? lab[k] : 0u;
+        bot[2 * k + 1] = (m & 8u) ? lab[k] : 0u;
+    }
+    if (EXACT || (G.flags & kVecOut)) {   // aligned label rows (uniform)
+        if (row_path) {
+            const uint32_t lim = min(4u, w - x0);
+            if (lim == 4) {
+                store_nt4(row0 + x0, top[0], top[1], top[2], top[3]);
+                if (two_rows) store_nt4(row0 + w + x0, bot[0], bot[1], bot[2], bot[3]);
+            } else {
+                store_words4(row0 + x0, top[0], top[1], top[2], top[3], 0, lim);
+                if (two_rows) store_words4(row0 + w + x0, bot[0], bot[1], bot[2], bot[3], 0, lim);
+            }
+        }
+    } else {   // aligned windows across lanes (shuffles by every lane of the wave)
+        const uint32_t lane = threadIdx.x & 63u;
+        const bool has_prev = lane != 0 && sp.bx >= FINAL_BLOCKS;                // the previous lane: this row, just left
+        const bool next_in_row = lane != 63 && sp.bx + 2 * FINAL_BLOCKS <= bw;   // the next lane: this row, just right
+        store_row_aligned(row0, x0, w, top[0], top[1], top[2], top[3], row_path, has_prev, next_in_row);
+        if (!CONN4)
+            store_row_aligned(row0 + w, x0, w, bot[0], bot[1], bot[2], bot[3], row_path && two_rows, has_prev, next_in_row);
+    }
+    if (sp.n && !row_path) {
+        {   // the blocks run into the next block row (odd bw): pixel by pixel
+            for (uint32_t k = 0; k < sp.n; ++k) {
+                uint32_t by, bx;
+                span_block(sp, k, bw, by, bx);
+                const uint32_t m = v[k] & 15u;
+                for (uint32_t dy = 0; dy < (CONN4 ? 1u : 2u); ++dy) {
+                    const uint32_t y = CONN4 ? by : 2 * by + dy;
+                    if (y >= h) continue;
+                    for (uint32_t dx = 0; dx < 2; ++dx) {
+                        const uint32_t x = 2 * bx + dx;
+                        if (x < w) G.labels[(size_t)y * w + x] = ((m >> (2 * dy + dx)) & 1u) ? lab[k] : 0u;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class Src, uint32_t MODE, bool CONN4, bool EXACT>
+__global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
+{
+    constexpr uint32_t CH = final_chunks<Src>();
+    uint32_t lw;
+    const Geom2D &G = src.by_final(blockIdx.x, lw);
+    // every chunk's block words are loaded before the first root chase
+    Span sp[CH];
+    uint32_t v[CH][FINAL_BLOCKS];
+#pragma unroll
+    for (uint32_t c = 0; c < CH; ++c) {
+        sp[c] = final_span(G, lw * CH + c);
+        load_span(G, sp[c], v[c]);
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < CH; ++c) final_span_labels<MODE, CONN4, EXACT>(G, sp[c], v[c]);
     // the tile and merge kernels have finished: the image's first workgroup sums its count
     if (lw == 0 && G.count && threadIdx.x < kCountSlots) count_reduce(G.slots, G.count);
 }
@@ -711,31 +924,24 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_final(Src src)
 // position of canonical numbering.  The keys kernel also clears the key bits
 // (canon.nwords words); the mark kernel then sets the bit of every root's key.
 
-template <class Src>
-__global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
+// One lane's offers for a span of blocks (see bleq2d_keys).
+__device__ __forceinline__ void keys_span(const Geom2D &G, const Span &sp)
 {
-    uint32_t lw;
-    const Geom2D &G = src.by_final(blockIdx.x, lw);
     const Canon &c = G.canon;
-    const uint64_t t = (uint64_t)lw * NT_FINAL + threadIdx.x;   // covers every word: bw * h / 64 < bw * bh * 2 / 2
-    if (t < c.nwords) c.bits[t] = 0ull;
-
     const uint32_t bw = G.bw;
-    const uint32_t by = lw / G.xgroups;
-    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
     uint32_t v[FINAL_BLOCKS] = {};
-    if (by < G.bh && bxs < bw)
-        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) v[k] = bxs + k < bw ? G.P[(size_t)by * bw + bxs + k] : 0u;
-
+    for (uint32_t k = 0; k < sp.n; ++k) v[k] = G.P[sp.k0 + k];
     // this lane's offers: (root, key) of its first candidate block, and of a
     // second one with another root.  Roots are never above parents (parent <=
     // self), so only a block whose parent lies in this block row can have its
     // root here: the others need no chase.  A root's key starts at its own
     // block's key, so only offers below the current key are made.
-    const uint32_t row0 = by * bw;
     uint32_t r0 = ~0u, k0 = ~0u, r1 = ~0u, k1 = ~0u, prev_p = ~0u, prev_r = 0, prev_k = 0;
 #pragma unroll
     for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+        uint32_t by, bx;
+        span_block(sp, k, bw, by, bx);
+        const uint32_t row0 = by * bw;
         const uint32_t m = v[k] & 15u, p = v[k] >> 4;
         if (!m || p < row0) continue;
         if (p != prev_p) {
@@ -745,7 +951,7 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
         }
         const uint32_t r = prev_r;
         if (r < row0) continue;   // not in its component's top block row
-        const uint32_t key = block_key(G, by, bxs + k, m);
+        const uint32_t key = block_key(G, by, bx, m);
         if (key >= prev_k) continue;
         if (r0 == ~0u || r0 == r) {
             r0 = r;
@@ -770,25 +976,38 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
     }
 }
 
+
+template <class Src>
+__global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
+{
+    constexpr uint32_t CH = final_chunks<Src>();
+    uint32_t lw;
+    const Geom2D &G = src.by_final(blockIdx.x, lw);
+#pragma unroll
+    for (uint32_t ch = 0; ch < CH; ++ch) {
+        const uint64_t t = (uint64_t)(lw * CH + ch) * NT_FINAL + threadIdx.x;   // covers every word: bw * h / 64 < bw * bh / 2
+        if (t < G.canon.nwords) G.canon.bits[t] = 0ull;
+        keys_span(G, final_span(G, lw * CH + ch));
+    }
+}
+
 // CONN4: the key is the root block index itself (a 1x2 root block holds the
 // component's first pixel, so block order is first-pixel order).
 template <class Src, bool CONN4>
 __global__ __launch_bounds__(NT_FINAL) void bleq2d_mark(Src src)
 {
+    constexpr uint32_t CH = final_chunks<Src>();
     uint32_t lw;
     const Geom2D &G = src.by_final(blockIdx.x, lw);
-    const uint32_t bw = G.bw;
-    const uint32_t by = lw / G.xgroups;
-    const uint32_t bxs = ((lw % G.xgroups) * NT_FINAL + threadIdx.x) * FINAL_BLOCKS;
-    if (by >= G.bh) return;
-#pragma unroll
-    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
-        const uint32_t b = by * bw + bxs + k;
-        if (bxs + k >= bw) break;
-        const uint32_t vb = G.P[b];
-        if ((vb & 15u) && (vb >> 4) == b) {   // a root
-            const uint32_t key = CONN4 ? b : G.canon.key[b];
-            if ((key >> 6) < G.canon.nwords) atomicOr(&G.canon.bits[key >> 6], 1ull << (key & 63u));
+    for (uint32_t ch = 0; ch < CH; ++ch) {
+        const Span sp = final_span(G, lw * CH + ch);
+        for (uint32_t k = 0; k < sp.n; ++k) {
+            const uint32_t b = sp.k0 + k;
+            const uint32_t vb = G.P[b];
+            if ((vb & 15u) && (vb >> 4) == b) {   // a root
+                const uint32_t key = CONN4 ? b : G.canon.key[b];
+                if ((key >> 6) < G.canon.nwords) atomicOr(&G.canon.bits[key >> 6], 1ull << (key & 63u));
+            }
         }
     }
 }
@@ -824,6 +1043,19 @@ template <uint32_t TBW, uint32_t TBH>
 struct Variant {
     static constexpr uint32_t THREADS = (TBW / LANE_BLOCKS) * TBH;
 
+    template <class Src, bool CONN4, bool EXACT>
+    static void launch_final(const Src &src, uint32_t total_final, hipStream_t s, uint32_t mode)
+    {
+        const dim3 grid(total_final), block(NT_FINAL);
+        switch (mode) {
+        case kModeRoot: hipLaunchKernelGGL((bleq2d_final<Src, kModeRoot, CONN4, EXACT>), grid, block, 0, s, src); break;
+        case kModeRef: hipLaunchKernelGGL((bleq2d_final<Src, kModeRef, CONN4, EXACT>), grid, block, 0, s, src); break;
+        case kModeCanon: hipLaunchKernelGGL((bleq2d_final<Src, kModeCanon, CONN4, EXACT>), grid, block, 0, s, src); break;
+        case kModeFirst: hipLaunchKernelGGL((bleq2d_final<Src, kModeFirst, CONN4, EXACT>), grid, block, 0, s, src); break;
+        default: hipLaunchKernelGGL((bleq2d_final<Src, kModeAny, CONN4, EXACT>), grid, block, 0, s, src); break;
+        }
+    }
+
     template <class Src, bool CONN4 = false>
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
                              KernelTimer *kt, bool flatten, uint32_t canon_chunks, uint32_t mode)
@@ -832,7 +1064,16 @@ struct Variant {
         const uint32_t merge_grid =
             (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + merge_jobs<Src>() - 1) / merge_jobs<Src>());
         const uint32_t border_grid = (uint32_t)(((uint64_t)total_tiles * Border<TBW, TBH>::JOBS + 255) / 256);
-        hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src, CONN4>), dim3(total_tiles), dim3(THREADS), 0, s, src);
+        bool exact = false;
+        if constexpr (Src::kSingle) exact = (src.g.flags & kExact) != 0;
+        if constexpr (Src::kSingle) {
+            if (exact)
+                hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src, CONN4, true>), dim3(total_tiles), dim3(THREADS), 0, s, src);
+            else
+                hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src, CONN4, false>), dim3(total_tiles), dim3(THREADS), 0, s, src);
+        } else {
+            hipLaunchKernelGGL((bleq2d_tile<TBW, TBH, Src, CONN4, false>), dim3(total_tiles), dim3(THREADS), 0, s, src);
+        }
         if (kt) kt->mark(s, "bleq2d_tile");
         hipLaunchKernelGGL((bleq2d_merge<TBW, TBH, Src, CONN4>), dim3(merge_grid), dim3(MERGE_THREADS), 0, s, src,
                            total_tiles);
@@ -857,12 +1098,13 @@ struct Variant {
             hipLaunchKernelGGL((bleq2d_canon_prefix<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
             if (kt) kt->mark(s, "canon_prefix");
         }
-        switch (mode) {
-        case kModeRoot: hipLaunchKernelGGL((bleq2d_final<Src, kModeRoot, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        case kModeRef: hipLaunchKernelGGL((bleq2d_final<Src, kModeRef, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        case kModeCanon: hipLaunchKernelGGL((bleq2d_final<Src, kModeCanon, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        case kModeFirst: hipLaunchKernelGGL((bleq2d_final<Src, kModeFirst, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
-        default: hipLaunchKernelGGL((bleq2d_final<Src, kModeAny, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src); break;
+        if constexpr (Src::kSingle) {
+            if (exact)
+                launch_final<Src, CONN4, true>(src, total_final, s, mode);
+            else
+                launch_final<Src, CONN4, false>(src, total_final, s, mode);
+        } else {
+            launch_final<Src, CONN4, false>(src, total_final, s, mode);
         }
         if (kt) kt->mark(s, "bleq2d_final");
         return hipGetLastError();
@@ -886,23 +1128,31 @@ hipError_t dispatch(uint32_t tile_id, const Src &src, uint32_t total_tiles, uint
     }
 }
 
+// the instantiations above follow the planner's tile table
+constexpr bool tile_is(uint32_t id, uint32_t bw, uint32_t bh) { return plan::kTiles[id].bw == bw && plan::kTiles[id].bh == bh; }
+static_assert(tile_is(0, 128, 32) && tile_is(1, 64, 64) && tile_is(2, 128, 64) && tile_is(3, 256, 32) &&
+                  tile_is(4, 256, 64) && tile_is(5, 512, 32) && tile_is(6, 128, 128),
+              "dispatch() and plan::kTiles disagree");
+static_assert(NT_FINAL * FINAL_BLOCKS == plan::kFinalWgBlocks && kCountBytes == plan::kCountBytes &&
+                  kCanonChunkWords == plan::kCanonChunkWords,
+              "kernel constants and the planner disagree");
+
 }  // namespace
 
-const TileShape2D kBleq2dTiles[kBleq2dTileCount] = {{128, 32}, {64, 64},  {128, 64}, {256, 32},
-                                                    {256, 64}, {512, 32}, {128, 128}};
-
-void bleq2d_geometry(Geom2D &g, uint32_t tile_id)
+void bleq2d_geometry(Geom2D &g, uint32_t tile_id, bool batch)
 {
-    const TileShape2D ts = kBleq2dTiles[tile_id < kBleq2dTileCount ? tile_id : 0];
-    g.bw = (g.w + 1u) / 2;
-    g.bh = (g.h + 1u) / 2;
-    g.tiles_x = (g.bw + ts.bw - 1) / ts.bw;
-    g.ntiles = g.tiles_x * ((g.bh + ts.bh - 1) / ts.bh);
-    g.xgroups = (g.bw + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS);
-    g.nfinal = g.xgroups * g.bh;
+    const plan::Geometry pg = plan::geometry2d(g.h, g.w, tile_id, batch ? plan::kBatchFinalChunks : 1u);
+    g.bw = pg.bw;
+    g.bh = pg.bh;
+    g.bdiv = plan::div_magic(g.bw);
+    g.tiles_x = pg.tiles_x;
+    g.ntiles = pg.ntiles;
+    g.nfinal = pg.nfinal;
     g.ref_bw = (uint32_t)((((uint64_t)g.w >> 5) << 5) + 32) / 2;   // LabelingTools.cpp:103
     g.flags &= kRefLabels | kKeys | kPixelKeys | kCanonLabels | kDropOrigin;
     if (g.pitch % 16 == 0 && reinterpret_cast<uintptr_t>(g.px) % 16 == 0) g.flags |= kVecIn;
+    if (g.w % 4 == 0 && reinterpret_cast<uintptr_t>(g.labels) % 16 == 0) g.flags |= kVecOut;
+    if ((g.flags & kVecIn) && (g.flags & kVecOut) && g.w % 32 == 0) g.flags |= kExact;
 }
 
 size_t bleq2d_workspace(uint32_t h, uint32_t w, bool canonical)
@@ -971,7 +1221,7 @@ hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hip
     g.bh = g.h;
     g.tiles_x = (g.bw + TBW - 1) / TBW;
     g.ntiles = g.tiles_x * ((g.bh + TBH - 1) / TBH);
-    g.nfinal = g.xgroups * g.bh;
+    g.nfinal = (uint32_t)(((uint64_t)g.bw * g.bh + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS));
     const bool canon = opt.mode == LBL_LABELS_CANONICAL;
     if (canon) g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256((size_t)g.bw * g.bh * sizeof(uint32_t)),
                                       (uint64_t)g.bw * g.bh, 0);
@@ -980,10 +1230,11 @@ hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hip
                                                 canon ? canon_chunks(g.canon.nwords) : 0u, canon ? kModeCanon : kModeFirst);
 }
 
-hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,
-                            uint32_t total_chunks, uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt)
+hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, const BatchTables &tables, uint32_t total_tiles,
+                            uint32_t total_final, uint32_t total_chunks, uint32_t tile_id, bool flatten, hipStream_t s,
+                            KernelTimer *kt)
 {
-    Many many{d_geoms, n};
+    Many many{d_geoms, n, tables};
     if (kt) kt->begin(s);
     return dispatch(tile_id, many, total_tiles, total_final, s, kt, flatten, total_chunks, kModeAny);
 }

@@ -2,6 +2,7 @@
 // the single-image entry points and the batch planner.
 #pragma once
 
+#include "batch_plan.hpp"
 #include "canon.hpp"
 
 namespace lbl {
@@ -13,6 +14,8 @@ enum : uint32_t {
     kPixelKeys = 16u,  // keys are pixel indices (else canonical bit positions)
     kCanonLabels = 32u,  // write canonical labels (batches: per image)
     kDropOrigin = 64u,   // pixel (0,0) reads as background (the reference's 2D label-equivalence values)
+    kVecOut = 128u,      // label rows start 16-byte aligned (w % 4 == 0 and base)
+    kExact = 256u,       // kVecIn, kVecOut and w % 32 == 0: every lane's segments and stores are whole
 };
 
 // One image as the kernels see it (also the element of a batch's device table).
@@ -24,8 +27,9 @@ struct Geom2D {
     uint32_t *slots;       // kCountBytes of partial counts (workspace, before P; zero between runs)
     uint64_t pitch;
     uint32_t h, w, bw, bh;
+    plan::DivMagic bdiv;        // division by bw (the final kernels' row of a raster block index)
     uint32_t tiles_x, ntiles;
-    uint32_t xgroups, nfinal;   // final-kernel workgroups: xgroups per block row, nfinal total
+    uint32_t nfinal;            // final-kernel workgroups: chunks of 512 blocks in raster order (plan::geometry2d)
     uint32_t tile_base;         // first global tile id of this image (batch)
     uint32_t final_base;        // first global final workgroup id (batch)
     uint32_t chunk_base;        // first global canonical scan chunk id (batch)
@@ -34,19 +38,24 @@ struct Geom2D {
     Canon canon;                // canonical numbering state (kCanon)
 };
 
-struct TileShape2D {
-    uint32_t bw, bh;   // tile size in blocks
+using TileShape2D = plan::TileShape;
+constexpr uint32_t kBleq2dTileCount = plan::kTileCount;
+constexpr const TileShape2D *kBleq2dTiles = plan::kTiles;
+
+// Device tables of a batch: the image of every tile id, final workgroup id and
+// canonical scan chunk id (plan::Batch).
+struct BatchTables {
+    const uint32_t *tile_img, *final_img, *chunk_img;
 };
 
-constexpr uint32_t kBleq2dTileCount = 7;
-extern const TileShape2D kBleq2dTiles[kBleq2dTileCount];
-
 // Fill bw..nfinal and the alignment flags for a tile shape (px, labels, pitch,
-// h, w and the kRefLabels / kCanon flags must be set).
-void bleq2d_geometry(Geom2D &g, uint32_t tile_id);
+// h, w and the kRefLabels / kCanon flags must be set); batch: an image of a
+// batch (the batched kernels' final workgroups take kBatchFinalChunks chunks).
+void bleq2d_geometry(Geom2D &g, uint32_t tile_id, bool batch = false);
 
 // total_chunks: the images' canonical scan chunks (0: no image asks for canonical labels)
-hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, uint32_t total_tiles, uint32_t total_final,
-                            uint32_t total_chunks, uint32_t tile_id, bool flatten, hipStream_t s, KernelTimer *kt);
+hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, const BatchTables &tables, uint32_t total_tiles,
+                            uint32_t total_final, uint32_t total_chunks, uint32_t tile_id, bool flatten, hipStream_t s,
+                            KernelTimer *kt);
 
 }  // namespace lbl

@@ -122,7 +122,9 @@ __device__ __forceinline__ uint32_t block_key3(uint32_t h, uint32_t w, uint32_t 
     return voxel_key ? zy * w + 2 * bx + dx : zy * bw + bx;
 }
 
-template <uint32_t TBW, uint32_t TBY, uint32_t TBZ, bool VEC>
+// VEC: 16-byte vector loads of the rows; FULL: w % 32 == 0, so every lane's
+// segments are whole and the kernel holds no code for row ends.
+template <uint32_t TBW, uint32_t TBY, uint32_t TBZ, bool VEC, bool FULL>
 __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     const uint8_t *__restrict__ vox, uint32_t d, uint32_t h, uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
     uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P, uint8_t *__restrict__ M, uint32_t *count,
@@ -150,7 +152,7 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         const size_t plane = (size_t)h * w, off[4] = {0, w, plane, plane + w};
         const bool use[4] = {true, y1, z1, z1 && y1};
         uint32_t m[4];
-        rows_bits32<4>(r00, off, use, x0, w, VEC, m);
+        rows_bits32<4, FULL>(r00, off, use, x0, w, VEC, m);
         a = m[0]; b = m[1]; c = m[2]; e = m[3];
     }
     const uint32_t o = a | b | c | e;
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
         if (ra == rb || (ra == la && rb == lb)) return;
         la = ra;
         lb = rb;
-        if (!set.insert(ra, rb)) links += Forest::unite(P, ra, rb);   // no room: unite here
+        if (!set.insert(ra, rb)) links += Forest::unite_cached(P, ra, rb);   // no room: unite here
     };
 
     if (job && gid < bar_jobs) {
@@ -450,7 +452,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
     const uint32_t n = set.size();
     for (uint32_t i = threadIdx.x; i < n; i += MERGE_THREADS) {
         const uint2 pr = set.pair(i);
-        links += Forest::unite(P, pr.x, pr.y);
+        links += Forest::unite_cached(P, pr.x, pr.y);
     }
     if (count) count_links(slots, links);   // kernel-uniform branch
 }
@@ -734,12 +736,15 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
 
     if (kt) kt->begin(s);
     constexpr uint32_t NT_TILE = Tile3<TX, TY, TZ>::THREADS;
-    if (vec_in)
-        hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
-                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
+    if (vec_in && im.w % 32 == 0)
+        hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true, true>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d,
+                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
+    else if (vec_in)
+        hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d,
+                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
     else
-        hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d, im.h,
-                           im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
+        hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d,
+                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
     if (kt) kt->mark(s, "bleq3d_tile");
     hipLaunchKernelGGL(bleq3d_merge, dim3(merge_grid), dim3(MERGE_THREADS), 0, s, P, M, g, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_merge");

@@ -1,5 +1,6 @@
 // canon.hip -- the scan behind canonical label numbering: set bits per chunk of
 // the key bit array, then per word the set bits before it.
+#include "batch_plan.hpp"
 #include "canon.hpp"
 
 namespace lbl {
@@ -22,9 +23,7 @@ uint64_t chunks_of(uint64_t nwords) { return (nwords + kCanonChunkWords - 1) / k
 
 size_t canon_workspace(uint64_t nkeys, uint64_t nnodes)
 {
-    const uint64_t nw = words_of(nkeys);
-    return align256(nnodes * sizeof(uint32_t)) + align256(nw * sizeof(unsigned long long)) +
-           align256(nw * sizeof(uint32_t)) + align256(chunks_of(nw) * sizeof(uint32_t));
+    return plan::canon_bytes(nkeys, nnodes);   // the layout below, as the batch planner computes it
 }
 
 Canon canon_layout(void *base, uint64_t nkeys, uint64_t nnodes)

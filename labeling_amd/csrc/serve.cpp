@@ -28,6 +28,7 @@ struct HipReq : serve::Req {
     uint64_t elems() const { return (uint64_t)(r.d ? r.d : 1) * r.h * r.w; }
     uint64_t in_bytes() const { return is3d() ? align256(elems()) : align256((uint64_t)r.h * ((r.w + 15u) & ~15u)); }
     uint64_t lab_bytes() const { return align256(elems() * 4); }
+    uint64_t table_bytes = 0;   // fusable: its share of the batch id tables, for any tile shape
     uint64_t ws_bytes() const
     {
         const bool c = canonical();
@@ -35,7 +36,20 @@ struct HipReq : serve::Req {
             return align256(r.alg == LBL_BLEQ ? bleq3d_workspace(r.d, r.h, r.w, c) : lbeq3d_workspace(r.d, r.h, r.w, c));
         return align256(r.alg == LBL_BLEQ ? bleq2d_workspace(r.h, r.w, c) : lbeq2d_workspace(r.h, r.w, c));
     }
+    uint64_t data_bytes() const { return in_bytes() + lab_bytes() + ws_bytes(); }
 };
+
+// Words of the batch id tables one 2D image adds (its tiles and final
+// workgroups), for the tile shape with the most tiles.
+uint64_t table_words_of(uint32_t h, uint32_t w)
+{
+    uint64_t m = 0;
+    for (uint32_t t = 0; t < plan::kTileCount; ++t) {
+        const plan::Geometry g = plan::geometry2d(h, w, t, plan::kBatchFinalChunks);
+        m = std::max<uint64_t>(m, (uint64_t)g.ntiles + g.nfinal);
+    }
+    return m;
+}
 
 // Arena bytes of a group beyond its requests: the batch table, then n counts
 // and the status word.
@@ -49,6 +63,8 @@ struct Slot {
     hipEvent_t done = nullptr;
     Geom2D *h_geoms = nullptr;     // pinned
     uint32_t *h_counts = nullptr;  // pinned: n counts + status word
+    uint32_t *h_tables = nullptr;  // pinned: batch id tables (grown on demand)
+    uint64_t tables_cap = 0;       // words
 };
 
 class HipBackend final : public serve::ServeBackend {
@@ -84,44 +100,69 @@ public:
         if (!ok(hipMemsetAsync(d_counts, 0, 4ull * n + 4, s))) return err;
 
         if (n > 1 || g.reqs[0]->fusable) {
-            // fused 2D bleq: per-image geometry table, then the batched kernels,
-            // in the launch configuration tuned for the group's largest request
+            // fused 2D bleq: per-image geometry table and id tables (batch_plan.hpp),
+            // then the batched kernels, in the launch configuration tuned for the
+            // group's largest request.  Arena: [Geom2D table][counts][id tables][requests]
             const HipReq *lead = static_cast<const HipReq *>(g.reqs[0]);
             for (serve::Req *r : g.reqs)
                 if (r->r.h * (uint64_t)r->r.w > lead->r.h * (uint64_t)lead->r.w) lead = static_cast<const HipReq *>(r);
             const uint32_t tile = lead->batch_tile;
-            uint32_t tb = 0, fb = 0;
+            std::vector<uint32_t> hs(n), ws(n);
+            uint64_t table_room = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                hs[i] = g.reqs[i]->r.h;
+                ws[i] = g.reqs[i]->r.w;
+                table_room += static_cast<const HipReq *>(g.reqs[i])->table_bytes;
+            }
+            plan::Batch pl;
+            if (!plan::plan_batch(hs.data(), ws.data(), n, tile, false, pl) || pl.table_words() * 4 > table_room) {
+                ok(hipErrorInvalidValue);
+                return err;
+            }
+            if (pl.table_words() > sl.tables_cap) {   // grow the slot's pinned staging
+                if (sl.h_tables) (void)hipHostFree(sl.h_tables);
+                sl.h_tables = nullptr;
+                sl.tables_cap = 0;
+                if (!ok(hipHostMalloc(reinterpret_cast<void **>(&sl.h_tables), pl.table_words() * 4, 0))) return err;
+                sl.tables_cap = pl.table_words();
+            }
+            uint32_t *d_tab = reinterpret_cast<uint32_t *>(base + cur);
+            cur += table_room;
+            std::copy(pl.tile_img.begin(), pl.tile_img.end(), sl.h_tables + pl.tile_tab());
+            std::copy(pl.final_img.begin(), pl.final_img.end(), sl.h_tables + pl.final_tab());
             for (uint32_t i = 0; i < n; ++i) {
                 const HipReq *rq = static_cast<const HipReq *>(g.reqs[i]);
                 const lbl_request &q = rq->r;
                 const uint32_t pitch = (q.w + 15u) & ~15u;
                 uint8_t *d_in = base + cur;
                 uint32_t *d_lab = reinterpret_cast<uint32_t *>(d_in + rq->in_bytes());
-                uint8_t *ws = reinterpret_cast<uint8_t *>(d_lab) + rq->lab_bytes();
-                cur += rq->bytes;
+                uint8_t *ws_i = reinterpret_cast<uint8_t *>(d_lab) + rq->lab_bytes();
+                cur += rq->data_bytes();
                 if (!ok(upload2d(d_in, pitch, q.px, q.w, q.w, q.h, reinterpret_cast<uint8_t *>(d_lab), rq->lab_bytes(), s)))
                     return err;
-                if (!ok(hipMemsetAsync(ws, 0, kCountBytes, s))) return err;   // count slots start at zero
+                if (!ok(hipMemsetAsync(ws_i, 0, kCountBytes, s))) return err;   // count slots start at zero
                 Geom2D &gm = sl.h_geoms[i];
                 gm = Geom2D{};
                 gm.px = d_in;
                 gm.labels = d_lab;
                 gm.count = d_counts + i;
-                gm.slots = reinterpret_cast<uint32_t *>(ws);
-                gm.P = reinterpret_cast<uint32_t *>(ws + align256(kCountBytes));
+                gm.slots = reinterpret_cast<uint32_t *>(ws_i);
+                gm.P = reinterpret_cast<uint32_t *>(ws_i + align256(kCountBytes));
                 gm.pitch = pitch;
                 gm.h = q.h;
                 gm.w = q.w;
                 gm.flags = rq->mode == LBL_LABELS_REFERENCE ? kRefLabels : 0u;
-                bleq2d_geometry(gm, tile);
-                gm.tile_base = tb;
-                gm.final_base = fb;
-                tb += gm.ntiles;
-                fb += gm.nfinal;
+                bleq2d_geometry(gm, tile, true);
+                gm.tile_base = pl.images[i].tile_base;
+                gm.final_base = pl.images[i].final_base;
             }
             Geom2D *d_geoms = reinterpret_cast<Geom2D *>(base);
             if (!ok(hipMemcpyAsync(d_geoms, sl.h_geoms, sizeof(Geom2D) * n, hipMemcpyHostToDevice, s))) return err;
-            if (!kernels(fail_kernels ? hipSuccess : bleq2d_run_batch(d_geoms, n, tb, fb, 0, tile, lead->batch_flatten, s, nullptr)))
+            if (!ok(hipMemcpyAsync(d_tab, sl.h_tables, pl.table_words() * 4, hipMemcpyHostToDevice, s))) return err;
+            const BatchTables tabs{d_tab + pl.tile_tab(), d_tab + pl.final_tab(), d_tab + pl.chunk_tab()};
+            if (!kernels(fail_kernels ? hipSuccess
+                                      : bleq2d_run_batch(d_geoms, n, tabs, pl.total_tiles, pl.total_final, 0, tile,
+                                                         lead->batch_flatten, s, nullptr)))
                 return err;
             for (uint32_t i = 0; i < n; ++i) {
                 const lbl_request &q = g.reqs[i]->r;
@@ -273,7 +314,8 @@ extern "C" lbl_status lbl_server_submit(lbl_server *srv, const lbl_request *req,
     }
     // fused launches: 2D bleq in root or reference values (canonical numbering runs per request)
     r->fusable = !r->is3d() && q.alg == LBL_BLEQ && !r->canonical();
-    r->bytes = r->in_bytes() + r->lab_bytes() + r->ws_bytes();
+    r->table_bytes = r->fusable ? align256(table_words_of(q.h, q.w) * 4) : 0;
+    r->bytes = r->data_bytes() + r->table_bytes;
     return srv->sched->submit(r, ticket);
 }
 
@@ -316,6 +358,7 @@ extern "C" void lbl_server_destroy(lbl_server *srv)
         if (s.stream) (void)hipStreamDestroy(s.stream);
         if (s.h_geoms) (void)hipHostFree(s.h_geoms);
         if (s.h_counts) (void)hipHostFree(s.h_counts);
+        if (s.h_tables) (void)hipHostFree(s.h_tables);
     }
     if (be.arena_base) (void)hipFree(be.arena_base);
     delete srv->sched;

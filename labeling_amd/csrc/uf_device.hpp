@@ -34,6 +34,15 @@ __device__ __forceinline__ uint32_t dev_load(const uint32_t *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A load that may be served by the caches of this CU / XCD and so may return
+// an earlier value of the location than other CUs have written (a plain
+// global_load, no device-coherence bits); atomic only so that concurrent
+// writes are not a data race.
+__device__ __forceinline__ uint32_t cached_load(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 __device__ __forceinline__ uint32_t dev_min(uint32_t *p, uint32_t v)
 {
     return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -56,27 +65,52 @@ template <int SHIFT>
 struct PackedForest {
     static constexpr uint32_t kLowMask = (1u << SHIFT) - 1u;
 
-    // Root of x by a read-only chase; the starting element is then pointed at
-    // the root, so the next find from it (typically another border union of
-    // the same tile-local root) takes one hop.  (Halving every visited entry
-    // costs a device-scope atomic per hop, and the entries near a large
-    // component's root are visited by thousands of finds at once.)
-    __device__ static uint32_t find(uint32_t *w, uint32_t x)
+    // Root of x by a chase with cached loads, which may return earlier parents:
+    // every value a parent entry ever held is an element of the node's final
+    // set and below it, so the chase ends at an element of x's set (possibly a
+    // root that has been linked since), and pointing x at it keeps parent <=
+    // self and the partition.  Faster than device-coherent loads, which miss
+    // the XCD's L2.
+    __device__ static uint32_t find_cached(uint32_t *w, uint32_t x)
     {
         const uint32_t x0 = x;
-        const uint32_t v0 = dev_load(&w[x]);
+        const uint32_t v0 = cached_load(&w[x]);
         uint32_t v = v0, root;
         for (;;) {
             const uint32_t p = v >> SHIFT;
             if (p == x) { root = x; break; }
-            const uint32_t vp = dev_load(&w[p]);
+            const uint32_t vp = cached_load(&w[p]);
             const uint32_t gp = vp >> SHIFT;
             if (gp == p) { root = p; break; }
             x = gp;
-            v = dev_load(&w[x]);
+            v = cached_load(&w[x]);
         }
         if (root != x0 && root != (v0 >> SHIFT)) dev_min(&w[x0], (root << SHIFT) | (v0 & kLowMask));
         return root;
+    }
+
+    // Unite the sets of a and b; true iff this call made the link that joined
+    // two different sets (exactly one call per merge of two sets returns
+    // true).  Finds by find_cached; whether a candidate root still is one is
+    // checked with a device-coherent load; if it is not, the union continues
+    // from its fresh parent (finding it again could return the same stale
+    // root forever).  Each pass either ends or strictly lowers max(a, b).
+    __device__ static bool unite_cached(uint32_t *w, uint32_t a, uint32_t b)
+    {
+        for (;;) {
+            a = find_cached(w, a);
+            b = find_cached(w, b);
+            if (a == b) return false;
+            if (a < b) { uint32_t t = a; a = b; b = t; }
+            const uint32_t va = dev_load(&w[a]);
+            if ((va >> SHIFT) != a) {
+                a = va >> SHIFT;
+                continue;
+            }
+            const uint32_t old = dev_min(&w[a], (b << SHIFT) | (va & kLowMask));
+            if ((old >> SHIFT) == a) return true;
+            a = old >> SHIFT;
+        }
     }
 
     // Root of x when nothing writes the forest any more (plain cached loads).
@@ -90,22 +124,6 @@ struct PackedForest {
         return x;
     }
 
-    // Unite the sets of a and b; true iff this call made the link that joined
-    // two different sets (exactly one call per merge of two sets returns true).
-    __device__ static bool unite(uint32_t *w, uint32_t a, uint32_t b)
-    {
-        for (;;) {
-            a = find(w, a);
-            b = find(w, b);
-            if (a == b) return false;
-            if (a < b) { uint32_t t = a; a = b; b = t; }
-            uint32_t va = dev_load(&w[a]);
-            if ((va >> SHIFT) != a) continue;   // a was linked meanwhile: re-find
-            uint32_t old = dev_min(&w[a], (b << SHIFT) | (va & kLowMask));
-            if ((old >> SHIFT) == a) return true;
-            a = old >> SHIFT;
-        }
-    }
 };
 
 // ---- label forest: label = index + 1, 0 = background -----------------------
@@ -328,12 +346,30 @@ __device__ __forceinline__ uint32_t nz_bits16(uint4 q)
 __device__ __forceinline__ uint32_t hibit(uint32_t v) { return 31u - __builtin_clz(v); }   // v != 0
 __device__ __forceinline__ uint32_t upto(uint32_t x) { return (2u << x) - 1u; }           // bits 0..x
 
-// 32 pixels of one row as a bit mask (bit k = column x0 + k), zero past the width.
+// 32 pixels of one row as a bit mask (bit k = column x0 + k), zero past the
+// width; x0 < w.  vec: row + x0 is 16-byte aligned.  Then the segment is read
+// as its (at most two) 16-byte chunks holding pixels of the row, and the bits
+// of columns >= w are masked off: an aligned 16-byte chunk never crosses a
+// page, so the bytes it holds after the row's last pixel can be read without
+// a fault (they belong to the padding, the next row or the end of the
+// buffer's last page).  The second chunk, when wholly past the row end, is
+// not read.
+__device__ __forceinline__ uint32_t tail_mask(uint32_t x0, uint32_t w)
+{
+    return w - x0 >= 32 ? ~0u : (1u << (w - x0)) - 1u;
+}
+
+// FULL: every segment is whole (the image width is a multiple of 32), so no
+// row end is handled.
+template <bool FULL = false>
 __device__ __forceinline__ uint32_t row_bits32(const uint8_t *row, uint32_t x0, uint32_t w, bool vec)
 {
-    if (vec && x0 + 32 <= w) {
+    if (vec) {
         const uint4 *q = reinterpret_cast<const uint4 *>(row + x0);
-        return nz_bits16(q[0]) | (nz_bits16(q[1]) << 16);
+        if (FULL) return nz_bits16(q[0]) | (nz_bits16(q[1]) << 16);
+        const uint32_t lo = nz_bits16(q[0]);
+        const uint32_t hi = x0 + 16 < w ? nz_bits16(q[1]) : 0u;   // a branch, not an address select
+        return (lo | (hi << 16)) & tail_mask(x0, w);
     }
     uint32_t m = 0;
     for (uint32_t k = 0; k < 32 && x0 + k < w; ++k) m |= (uint32_t)(row[x0 + k] != 0) << k;
@@ -345,17 +381,23 @@ __device__ __forceinline__ uint32_t row_bits32(const uint8_t *row, uint32_t x0, 
 // is converted, so a lane waits for one memory round trip instead of N in a
 // row; an unused row re-reads row0 (cached) instead of branching around its
 // load.
-template <uint32_t N>
+template <uint32_t N, bool FULL = false>
 __device__ __forceinline__ void rows_bits32(const uint8_t *row0, const size_t (&off)[N], const bool (&use)[N], uint32_t x0,
                                             uint32_t w, bool vec, uint32_t (&m)[N])
 {
-    if (vec && x0 + 32 <= w) {
+    if (vec) {
+        // the second chunk is read unless wholly past the row end (a branch
+        // on the lane's own condition, uniform in waves that hold no row end:
+        // a select of its address would split its load into dword loads)
+        // each row's two chunks are requested back to back
+        const bool second = FULL || x0 + 16 < w;
         uint4 lo[N], hi[N];
 #pragma unroll
         for (uint32_t i = 0; i < N; ++i) {
             const uint4 *q = reinterpret_cast<const uint4 *>(row0 + (use[i] ? off[i] : 0) + x0);
             lo[i] = q[0];
-            hi[i] = q[1];
+            hi[i] = make_uint4(0u, 0u, 0u, 0u);
+            if (second) hi[i] = q[1];
         }
         // Needed here, unconditionally: keeps the compiler from sinking a
         // load into the branch of its `use` test, behind the other rows' waits.
@@ -366,10 +408,15 @@ __device__ __forceinline__ void rows_bits32(const uint8_t *row0, const size_t (&
         }
 #pragma unroll
         for (uint32_t i = 0; i < N; ++i) m[i] = use[i] ? nz_bits16(lo[i]) | (nz_bits16(hi[i]) << 16) : 0u;
+        if (!FULL && x0 + 32 > w) {   // the row's last segment (a branch: uniform in waves that hold no row end)
+            const uint32_t keep = tail_mask(x0, w);
+#pragma unroll
+            for (uint32_t i = 0; i < N; ++i) m[i] &= keep;
+        }
         return;
     }
 #pragma unroll
-    for (uint32_t i = 0; i < N; ++i) m[i] = use[i] ? row_bits32(row0 + off[i], x0, w, false) : 0u;
+    for (uint32_t i = 0; i < N; ++i) m[i] = use[i] ? row_bits32<false>(row0 + off[i], x0, w, false) : 0u;
 }
 
 // ---- run forests of bit-mask segments ------------------------------------------

@@ -1,24 +1,40 @@
 // rem_interleave_check.cpp -- CPU model check of the concurrent union-find
 // unions used on the device, at the granularity of single shared-memory
 // operations.  Each simulated lane runs its unions as a state machine whose
-// every step is exactly one load or one atomicMin on the shared parent array
-// (sequentially consistent, as LDS within a workgroup); a random scheduler
-// interleaves the steps of all lanes.  After all lanes finish, the forest's
-// partition must equal the union of the initial forest and all requested
-// pairs, and parents must never exceed their node.
+// every step is exactly one load or one atomicMin on the shared parent array;
+// a random scheduler interleaves the steps of all lanes.  After all lanes
+// finish, the forest's partition must equal the union of the initial forest
+// and all requested pairs, parents must never exceed their node, every lane
+// must terminate, and (global unites) the number of unions reporting a link
+// must equal the drop in the number of roots (the component count relies on it).
+//
+// Memory models for loads (atomicMin always acts on the latest value):
+//   seq-cst    every load returns the latest value (LDS within a workgroup);
+//   coherent   a load may return an earlier value of its location, but never
+//              one older than the lane has already seen there (relaxed atomic
+//              loads of global memory);
+//   cached     as coherent for device-coherent loads; a plain load may return
+//              ANY earlier value of its location (a line kept in a non-coherent
+//              cache for the whole kernel, after the lane saw newer values).
 //
 // Variants:
 //   lds     the tile kernels' loop: lds_rem_step (Rem's algorithm with
 //           splicing, uf_device.hpp) until done, then both ORIGINAL nodes are
-//           pointed at the meeting node (atomicMin).
-//   global  the global-forest unite that was tried and removed (Rem's
-//           splicing, then the meeting node chased to its root, then the
-//           starting nodes pointed at that root, guarded by their parents
-//           as first read), and its "linked" result used for component counts.
-//   find    the shipped global unite: read-only finds to both roots, link the
-//           larger root under the smaller by atomicMin, retry.
-// Usage: rem_interleave_check [trials]; prints a counterexample on failure.
-// Exit code 0 = every variant behaved as expected (see main).
+//           pointed at the meeting node.
+//   rem     the global-forest unite that was tried and removed (Rem's
+//           splicing, the meeting node chased to its root, the starting nodes
+//           pointed at it when their first-read parents are larger).
+//   find    the global unite before cached loads: finds that chase by
+//           device-coherent loads and point their start at the root found;
+//           link the larger root under the smaller by atomicMin; a root that
+//           was linked meanwhile is found again.
+//   cfind   the merges' unite (PackedForest::unite_cached): find chasing by
+//           plain (cacheable) loads; the root check is a device-coherent load,
+//           and a node found not to be a root continues from its fresh parent
+//           instead of being found again (which could see the same stale root
+//           forever).
+// Usage: rem_interleave_check [trials]; exit code 0 = the variants the device
+// uses are exact under their memory model (lds: seq-cst; cfind: all three).
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
@@ -30,16 +46,13 @@
 namespace {
 
 constexpr int kMaxN = 16;
+enum Model { kSeqCst, kCoherent, kCached };
+enum Kind { kPlain, kFresh };
 
-// The shared parent array.  Sequentially consistent by default (LDS).  With
-// `stale`, a load may return any earlier value of its location that is not
-// older than what the loading lane has already observed there (per-location
-// coherence only, what relaxed atomic loads of global memory guarantee);
-// atomicMin always acts on the latest value.
 struct Shared {
     uint32_t par[kMaxN];
-    int n;
-    bool stale = false;
+    int n = 0;
+    Model model = kSeqCst;
     std::mt19937_64 *rng = nullptr;
     std::vector<uint32_t> hist[kMaxN];   // every value the location has held, oldest first
     void init(const uint32_t *p, int n_)
@@ -50,10 +63,15 @@ struct Shared {
             hist[i].assign(1, p[i]);
         }
     }
-    uint32_t load(uint32_t i, size_t &seen)
+    uint32_t load(uint32_t i, size_t &seen, Kind kind)
     {
-        size_t v = hist[i].size() - 1;
-        if (stale && v > seen) v = seen + (*rng)() % (v - seen + 1);
+        const size_t last = hist[i].size() - 1;
+        size_t v = last;
+        if (model == kCached && kind == kPlain) {
+            v = (*rng)() % (last + 1);   // any earlier value; does not move `seen`
+            return hist[i][v];
+        }
+        if (model != kSeqCst && last > seen) v = seen + (*rng)() % (last - seen + 1);
         if (v < seen) v = seen;
         seen = v;
         return hist[i][v];
@@ -70,36 +88,38 @@ struct Shared {
     }
 };
 
-enum Variant { kLds, kGlobal, kFind };
+enum Variant { kLds, kRem, kFind, kCFind, kVariants };
+const char *kNames[kVariants] = {"lds", "rem", "find", "cfind"};
 
 // One lane running a list of unions, one shared-memory operation per step().
 struct Lane {
     std::vector<std::pair<uint32_t, uint32_t>> jobs;
     size_t next = 0;
-    int pc = 0;   // 0 = start next job
+    int pc = 0;   // 0 = start the next job
     uint32_t oa = 0, ob = 0, x = 0, y = 0, px = 0, py = 0, top = 0, v0x = 0, v0y = 0, chase = 0;
-    int links = 0;   // "linked" results (global variant) / root links (find variant)
+    // find sub-machine: chases from fstart, result in froot; `ret` resumes the caller
+    uint32_t fstart = 0, fcur = 0, fv0 = 0, fv = 0, froot = 0, fa = 0, fb = 0;
+    int ret = 0;
+    int links = 0;
+    size_t seen[kMaxN] = {};
     bool done() const { return next >= jobs.size() && pc == 0; }
-
-    // find variant state
-    uint32_t fa = 0, fb = 0, fcur = 0;
-    size_t seen[kMaxN] = {};   // newest version of each location this lane has observed
-    uint32_t ld(Shared &s, uint32_t i) { return s.load(i, seen[i]); }
+    uint32_t ld(Shared &s, uint32_t i, Kind k) { return s.load(i, seen[i], k); }
     uint32_t mn(Shared &s, uint32_t i, uint32_t v) { return s.amin(i, v, seen[i]); }
 
     void step(Shared &s, Variant var)
     {
+        const Kind chase_kind = var == kCFind ? kPlain : kFresh;
         if (pc == 0) {
             oa = x = jobs[next].first;
             ob = y = jobs[next].second;
             ++next;
-            pc = var == kGlobal ? 10 : (var == kFind ? 30 : 1);
+            pc = var == kRem ? 10 : (var == kLds ? 1 : 30);
         }
         switch (pc) {
-        // ---- lds: lds_rem_step loop, then compress both original nodes to the meeting node
-        case 1: px = ld(s, x); pc = 2; return;
+        // ---- lds
+        case 1: px = ld(s, x, kFresh); pc = 2; return;
         case 2:
-            py = ld(s, y);
+            py = ld(s, y, kFresh);
             if (px == py) { top = px; pc = 4; return; }
             if (px < py) { std::swap(x, y); std::swap(px, py); }
             pc = 3;
@@ -114,9 +134,9 @@ struct Lane {
         case 4: mn(s, oa, top); pc = 5; return;
         case 5: mn(s, ob, top); pc = 0; return;
 
-        // ---- global (removed): initial loads, splicing loop, chase, guarded compression
-        case 10: v0x = ld(s, oa); pc = 11; return;
-        case 11: v0y = ld(s, ob); px = v0x; py = v0y; pc = 12; [[fallthrough]];
+        // ---- rem (removed global variant)
+        case 10: v0x = ld(s, oa, kFresh); pc = 11; return;
+        case 11: v0y = ld(s, ob, kFresh); px = v0x; py = v0y; pc = 12; [[fallthrough]];
         case 12:
             if (px == py) { top = px; pc = 15; return; }
             if (px < py) { std::swap(x, y); std::swap(px, py); }
@@ -129,9 +149,9 @@ struct Lane {
             pc = 14;
             return;
         }
-        case 14: px = ld(s, x); pc = 16; return;
-        case 16: py = ld(s, y); pc = 12; return;
-        case 15: chase = ld(s, top); pc = 17; return;   // chase top to its root
+        case 14: px = ld(s, x, kFresh); pc = 16; return;
+        case 16: py = ld(s, y, kFresh); pc = 12; return;
+        case 15: chase = ld(s, top, kFresh); pc = 17; return;
         case 17:
             if (chase != top) { top = chase; pc = 15; return; }
             pc = 18;
@@ -139,33 +159,50 @@ struct Lane {
         case 18: if (v0x > top) mn(s, oa, top); pc = 19; return;
         case 19: if (v0y > top) mn(s, ob, top); pc = 0; return;
 
-        // ---- find: read-only finds (one load per step), link the larger root
-        case 30: fcur = x; pc = 31; [[fallthrough]];
-        case 31: {
-            const uint32_t p = ld(s, fcur);
-            if (p != fcur) { fcur = p; return; }
-            fa = fcur;
-            fcur = y;
-            pc = 32;
-            return;
-        }
-        case 32: {
-            const uint32_t p = ld(s, fcur);
-            if (p != fcur) { fcur = p; return; }
-            fb = fcur;
+        // ---- find / cfind: unite(x, y)
+        case 30: fstart = x; ret = 31; pc = 50; return step(s, var);   // a = find(x)
+        case 31: fa = froot; fstart = y; ret = 32; pc = 50; return step(s, var);   // b = find(y)
+        case 32:
+            fb = froot;
             if (fa == fb) { pc = 0; return; }
             if (fa < fb) std::swap(fa, fb);
             pc = 33;
             return;
+        case 33: {   // is a still a root?  (device-coherent load)
+            const uint32_t va = ld(s, fa, kFresh);
+            if (va != fa) {
+                if (var == kCFind) { x = va; y = fb; pc = 30; return; }   // continue from the fresh parent
+                x = fa; y = fb; pc = 30;   // find a again
+                return;
+            }
+            pc = 34;
+            return;
         }
-        case 33: {
+        case 34: {
             const uint32_t old = mn(s, fa, fb);
             if (old == fa) { ++links; pc = 0; return; }
-            x = old;   // fa was linked meanwhile: unite old and fb again
+            x = old;
             y = fb;
             pc = 30;
             return;
         }
+        // find(fstart) as PackedForest::find: v0 = w[x0]; chase x -> p -> gp;
+        // then point x0 at the root unless it is x0 or its first-read parent
+        case 50: fcur = fstart; fv0 = fv = ld(s, fstart, chase_kind); pc = 51; return;
+        case 51: {
+            const uint32_t p = fv;
+            if (p == fcur) { froot = fcur; pc = 54; return; }
+            const uint32_t vp = ld(s, p, chase_kind);
+            if (vp == p) { froot = p; pc = 54; return; }
+            fcur = vp;
+            pc = 52;
+            return;
+        }
+        case 52: fv = ld(s, fcur, chase_kind); pc = 51; return;
+        case 54:
+            if (froot != fstart && froot != fv0) mn(s, fstart, froot);
+            pc = ret;
+            return;
         default: std::abort();
         }
     }
@@ -181,7 +218,7 @@ struct Result {
     bool partition_ok = true, invariant_ok = true, count_ok = true, terminated = true;
 };
 
-Result run_trial(std::mt19937_64 &rng, Variant var, bool stale, int n, int lanes, int jobs_per_lane,
+Result run_trial(std::mt19937_64 &rng, Variant var, Model model, int n, int lanes, int jobs_per_lane,
                  std::string *trace)
 {
     Shared s{};
@@ -189,11 +226,10 @@ Result run_trial(std::mt19937_64 &rng, Variant var, bool stale, int n, int lanes
     uint32_t init[kMaxN];
     for (int i = 0; i < n; ++i) init[i] = (i > 0 && rng() % 3 == 0) ? (uint32_t)(rng() % i) : (uint32_t)i;
     s.init(init, n);
-    s.stale = stale;
+    s.model = model;
     s.rng = &rng;
-    // reference partition
     uint32_t ref[kMaxN];
-    std::copy(s.par, s.par + n, ref);
+    std::copy(init, init + n, ref);
     auto rfind = [&](uint32_t a) {
         while (ref[a] != a) a = ref[a];
         return a;
@@ -204,12 +240,12 @@ Result run_trial(std::mt19937_64 &rng, Variant var, bool stale, int n, int lanes
     for (auto &l : L)
         for (int j = 0; j < jobs_per_lane; ++j) {
             uint32_t a = rng() % n, b = rng() % n;
-            if (var != kFind) {   // lanes unite nodes, which the tile kernels and merges start from
+            if (var == kLds || var == kRem || rng() % 2) {   // unions start at nodes (tile-local roots)
                 a = root_of(init, a);
                 b = root_of(init, b);
             }
             l.jobs.push_back({a, b});
-            uint32_t ra = rfind(a), rb = rfind(b);
+            const uint32_t ra = rfind(a), rb = rfind(b);
             if (ra != rb) ref[std::max(ra, rb)] = std::min(ra, rb);
         }
     Result r;
@@ -228,7 +264,7 @@ Result run_trial(std::mt19937_64 &rng, Variant var, bool stale, int n, int lanes
         L[k].step(s, var);
         for (int i = 0; i < n; ++i)
             if (s.par[i] > (uint32_t)i) r.invariant_ok = false;
-        if (++steps > 100000) {
+        if (++steps > 200000) {
             r.terminated = false;
             break;
         }
@@ -250,34 +286,35 @@ Result run_trial(std::mt19937_64 &rng, Variant var, bool stale, int n, int lanes
 int main(int argc, char **argv)
 {
     const long trials = argc > 1 ? std::atol(argv[1]) : 300000;
-    const char *names[] = {"lds", "global", "find"};
-    long fails[2][3][4] = {};
-    for (int st = 0; st < 2; ++st)
-        for (int v = 0; v < 3; ++v) {
+    const char *models[] = {"seq-cst", "coherent", "cached"};
+    long fails[3][kVariants][4] = {};
+    for (int m = 0; m < 3; ++m)
+        for (int v = 0; v < kVariants; ++v) {
             std::string example;
-            std::mt19937_64 rng(1234 + v + 10 * st);
+            std::mt19937_64 rng(1234 + v + 10 * m);
             for (long t = 0; t < trials; ++t) {
                 const int n = 4 + (int)(rng() % 9), lanes = 2 + (int)(rng() % 5), jobs = 1 + (int)(rng() % 3);
                 std::mt19937_64 keep = rng;
-                const Result r = run_trial(rng, (Variant)v, st != 0, n, lanes, jobs, nullptr);
+                const Result r = run_trial(rng, (Variant)v, (Model)m, n, lanes, jobs, nullptr);
                 const bool bad[4] = {!r.partition_ok, !r.invariant_ok, !r.count_ok, !r.terminated};
-                for (int k = 0; k < 4; ++k) fails[st][v][k] += bad[k];
+                for (int k = 0; k < 4; ++k) fails[m][v][k] += bad[k];
                 if ((bad[0] || bad[1] || bad[2] || bad[3]) && example.empty()) {
                     std::string tr;
-                    run_trial(keep, (Variant)v, st != 0, n, lanes, jobs, &tr);
+                    run_trial(keep, (Variant)v, (Model)m, n, lanes, jobs, &tr);
                     example = "n=" + std::to_string(n) + " lanes=" + std::to_string(lanes) + ": " + tr.substr(0, 1200);
                 }
             }
-            std::printf("%-6s %-9s trials %ld: lost unions %ld, parent>self %ld, link count wrong %ld, "
-                        "no termination %ld\n", names[v], st ? "stale" : "seq-cst", trials, fails[st][v][0],
-                        fails[st][v][1], fails[st][v][2], fails[st][v][3]);
+            std::printf("%-5s %-8s trials %ld: lost unions %ld, parent>self %ld, link count wrong %ld, "
+                        "no termination %ld\n", kNames[v], models[m], trials, fails[m][v][0], fails[m][v][1],
+                        fails[m][v][2], fails[m][v][3]);
             if (!example.empty()) std::printf("  first counterexample: %s\n", example.c_str());
         }
-    // required: the LDS loop exact under sequential consistency (LDS), the
-    // shipped global unite exact also with stale loads (global memory)
-    const bool lds_ok = !fails[0][0][0] && !fails[0][0][1] && !fails[0][0][3];
-    const bool find_ok = !fails[0][2][0] && !fails[0][2][1] && !fails[0][2][2] && !fails[0][2][3] &&
-                         !fails[1][2][0] && !fails[1][2][1] && !fails[1][2][2] && !fails[1][2][3];
-    std::printf("shipped variants %s\n", lds_ok && find_ok ? "exact" : "FAILED");
-    return lds_ok && find_ok ? 0 : 1;
+    auto exact = [&](int m, int v) {
+        return !fails[m][v][0] && !fails[m][v][1] && !fails[m][v][2] && !fails[m][v][3];
+    };
+    // the device's variants under the memory each one runs on
+    const bool ok = !fails[kSeqCst][kLds][0] && !fails[kSeqCst][kLds][1] && !fails[kSeqCst][kLds][3] &&
+                    exact(kSeqCst, kCFind) && exact(kCoherent, kCFind) && exact(kCached, kCFind);
+    std::printf("device variants %s\n", ok ? "exact" : "FAILED");
+    return ok ? 0 : 1;
 }

@@ -1,9 +1,9 @@
 """Model check of the concurrent unions on the CPU (tests/cpp/rem_interleave_check.cpp):
 the tile kernels' LDS loop (Rem's splicing, lds_rem_step) and the global
-merges' unite (read-only finds, root link by atomicMin) keep every union under
-random interleavings of single loads and atomicMins, with sequentially
-consistent memory and (for the global unite) with loads that may return stale
-values of their location."""
+merges' unite (finds by cacheable loads, root check by a device-coherent load,
+root link by atomicMin) keep every union under random interleavings of single
+loads and atomicMins: the LDS loop with sequentially consistent memory, the
+global unite also with loads that return any earlier value of their location."""
 import os
 import subprocess
 
@@ -18,4 +18,4 @@ def test_union_interleavings(tmp_path):
     r = subprocess.run([exe, "60000"], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "shipped variants exact" in r.stdout
+    assert "device variants exact" in r.stdout
