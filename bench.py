@@ -451,6 +451,7 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=24)
     ap.add_argument("--serve-outputs", type=int, default=16)
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB: only run a plain device copy (counter scale)")
+    ap.add_argument("--only3d", action="store_true", help="only the 3D section (counter runs); prints its JSON")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (every image is labelled on "
@@ -477,6 +478,10 @@ def main():
     import labeling_amd as L
 
     lab = L.Labeler(0)
+    if a.only3d:
+        print(json.dumps({"volume": bench_3d(L, torch, lab, a)}), flush=True)
+        lab.close()
+        return
     r2 = bench_2d(L, torch, lab, a)
     skip = set(filter(None, a.skip.split(",")))
     extra = {}

@@ -239,7 +239,10 @@ lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_image2d *images
 
 /* Times every tile shape and merge schedule of the 2D path on one device image
  * and records the fastest for images of its size class (floor(log2(h*w))) in
- * the context's tuning table, which later 2D calls of that class use. */
+ * the context's tuning table, which later 2D calls of that class use.  The
+ * timed launches run on the context's stream, after the work enqueued on
+ * `stream` (where d_px is ready) so far; on return `stream` waits for them
+ * (they write d_labels).  stream NULL: the context's stream. */
 lbl_status lbl_autotune2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
                           size_t pitch_bytes, uint32_t *d_labels, void *stream);
 

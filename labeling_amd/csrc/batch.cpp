@@ -377,7 +377,23 @@ extern "C" lbl_status lbl_autotune2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d
     if (((h + 1ull) / 2) * ((w + 1ull) / 2) > kMaxBlocks2D)
         return bfail(ctx, LBL_EUNSUPPORTED, "bleq 2D supports at most 2^28 2x2 blocks");
     DevGuard dg(ctx->device);
-    (void)stream;
+    // The timed launches run on the context's stream (its events time them),
+    // ordered after the caller's stream, on which the image is ready; the
+    // caller's stream then waits for them (they write d_labels).
+    struct EventGuard {
+        hipEvent_t e = nullptr;
+        ~EventGuard()
+        {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } order;
+    hipStream_t caller = static_cast<hipStream_t>(stream);
+    if (caller == ctx->stream) caller = nullptr;
+    if (caller) {
+        B_HIP(ctx, hipEventCreateWithFlags(&order.e, hipEventDisableTiming));
+        B_HIP(ctx, hipEventRecord(order.e, caller));
+        B_HIP(ctx, hipStreamWaitEvent(ctx->stream, order.e, 0));
+    }
     const size_t need = bleq2d_workspace(h, w);
     if (ctx->ws_bytes < need) {
         if (ctx->ws) {
@@ -409,6 +425,10 @@ extern "C" lbl_status lbl_autotune2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d
         }
     if (best < 0.f) return bfail(ctx, LBL_EDEVICE, "autotune: no configuration ran");
     ctx->tuning[TuneKey{(int)alg, 0, size_class_of((uint64_t)h * w)}] = bestv;
+    if (caller) {
+        B_HIP(ctx, hipEventRecord(order.e, ctx->stream));
+        B_HIP(ctx, hipStreamWaitEvent(caller, order.e, 0));
+    }
     return LBL_OK;
 }
 

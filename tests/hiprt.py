@@ -21,6 +21,7 @@ def lib():
         h.hipFree.argtypes = [vp]
         h.hipMemcpy.argtypes = [vp, vp, sz, i32]
         h.hipMemset.argtypes = [vp, i32, sz]
+        h.hipMemcpyAsync.argtypes = [vp, vp, sz, i32, vp]
         h.hipDeviceSynchronize.argtypes = []
         h.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
         h.hipStreamDestroy.argtypes = [vp]
@@ -77,6 +78,11 @@ def to_device(arr):
     b = DeviceBuffer(np.asarray(arr).nbytes)
     b.upload(arr)
     return b
+
+
+def copy_async(dst, src, nbytes, stream):
+    """Device-to-device copy enqueued on `stream` (a Stream)."""
+    check(lib().hipMemcpyAsync(dst, src, nbytes, D2D, stream.handle), "hipMemcpyAsync")
 
 
 def synchronize():

@@ -214,6 +214,35 @@ def test_autotune_then_label(tmp_path):
         lab.close()
 
 
+def test_autotune_orders_after_the_callers_stream():
+    """lbl_autotune2d(stream=S): the timed launches run after what S holds at
+    the call (here the image, copied in on S with no host wait), and S waits
+    for them on return (the labels they write, copied out on S)."""
+    from tests import hiprt
+
+    lab = L.Labeler(0)
+    s = hiprt.Stream()
+    try:
+        h, w = 2048, 2048
+        img = L.bernoulli((h, w), 0.5, seed=5)
+        d_src = hiprt.to_device(img)
+        d_px = hiprt.DeviceBuffer(h * w)
+        d_px.zero()
+        d_lab = hiprt.DeviceBuffer(h * w * 4)
+        d_out = hiprt.DeviceBuffer(h * w * 4)
+        d_out.zero()
+        hiprt.copy_async(d_px.addr, d_src.addr, h * w, s)
+        lab.autotune2d(d_px.addr, h, w, w, d_lab.addr, stream=s.handle.value)
+        hiprt.copy_async(d_out.addr, d_lab.addr, h * w * 4, s)
+        s.synchronize()
+        got = d_out.download((h, w), np.int32)
+        canon, m = O.bfs2d(img, 8)
+        np.testing.assert_array_equal(O.canonical(got)[0], canon)
+    finally:
+        s.destroy()
+        lab.close()
+
+
 def test_counts_identical_over_repeated_runs(labeler):
     """One hundred labelings of the 8192^2 Bernoulli p=0.5 golden input (one
     component spanning the image plus 220 k others: the heaviest union work
