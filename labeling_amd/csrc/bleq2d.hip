@@ -420,19 +420,23 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
     // components are counted at their roots (the merge subtracts its links)
     // every lane computes its words (selects only: no per-block branch; a
     // lane outside the image has no pixels, so no nodes and no roots, and
-    // stores nothing)
+    // stores nothing).  A background block (mask 0) carries the parent of the
+    // run before it (or 0), <= its own index: no reader uses the parent of a
+    // block whose mask is 0.
     const bool inside = by < bh && bxs < bw;
-    uint32_t out[LANE_BLOCKS], cur = 0, rootmask = 0;
+    uint32_t out[LANE_BLOCKS], cur = 0;
 #pragma unroll
     for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk) {
         const uint32_t m = ((a >> (2 * kk)) & 3u) | (((b >> (2 * kk)) & 3u) << 2);
         const bool node = (st & (3u << (2 * kk))) != 0;
-        const uint32_t own = g0 + kk;
-        const uint32_t nv = (v[kk] & kResolved) ? v[kk] : own;
+        const uint32_t nv = (v[kk] & kResolved) ? v[kk] : g0 + kk;
         cur = node ? nv : cur;
-        rootmask |= (node && nv == own) ? 1u << kk : 0u;
-        out[kk] = m ? (cur << 4) | m : own << 4;   // << 4 drops the tag bit
+        out[kk] = (cur << 4) | m;   // << 4 drops the tag bit
     }
+    // the tile-local roots: the nodes no chase started from (their entries
+    // still hold their own index; resolution writes only the chased nodes')
+    static_assert(LANE_BLOCKS < 32, "count_add_wave takes per-lane counts below 32");
+    const uint32_t rootmask = pair_any16(st) & ~pend;
     const uint32_t roots = __builtin_popcount(rootmask);
     // first-pixel keys start at the root block's own first pixel (global
     // roots are tile-local roots)
@@ -443,7 +447,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
             G.canon.key[g0 + kk] = block_key(G, by, bxs + kk, m);
         }
     store_block_words<LPB, EXACT>(G.P, (size_t)by * bw + bxs, bxs, inside, j, bw, out);
-    if (G.count) count_add(G.slots, roots);   // one image per workgroup: uniform branch
+    if (G.count) count_add_wave(G.slots, roots);   // one image per workgroup: uniform branch
 }
 
 // ---- tile-border block jobs (the flatten kernel) ------------------------------------

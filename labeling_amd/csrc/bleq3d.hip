@@ -257,11 +257,15 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         load_nodes();
     }
 
-    // block words (root, own index for background) and masks (bit dx + 2 dy + 4 dz);
-    // the tile's components are counted at their roots (the merge subtracts its links)
-    uint32_t roots = 0;
+    // block words (root; a background block, mask 0, carries the parent of the
+    // run before it, or 0: no reader uses it) and masks (bit dx + 2 dy + 4 dz);
+    // the tile's components are counted at their roots (the merge subtracts
+    // its links): the nodes no chase started from
+    static_assert(kLaneBlocks < 32, "count_add_wave takes per-lane counts below 32");
+    const uint32_t rootmask = pair_any16(st) & ~pend;
+    const uint32_t roots = __builtin_popcount(rootmask);
     if (bz < bd && by < bh && bxs < bw) {
-        uint32_t out[kLaneBlocks], mk[kLaneBlocks / 4] = {0, 0, 0, 0}, cur = 0, rootmask = 0;
+        uint32_t out[kLaneBlocks], mk[kLaneBlocks / 4] = {0, 0, 0, 0}, cur = 0;
         const auto mask_of = [&](uint32_t k) {
             return ((a >> (2 * k)) & 3u) | (((b >> (2 * k)) & 3u) << 2) | (((c >> (2 * k)) & 3u) << 4) |
                    (((e >> (2 * k)) & 3u) << 6);
@@ -270,14 +274,11 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         for (uint32_t k = 0; k < kLaneBlocks; ++k) {   // selects only: no per-block branch
             const uint32_t m = mask_of(k);
             const bool node = (st & (3u << (2 * k))) != 0;
-            const uint32_t own = g0 + k;
-            const uint32_t nv = (v[k] & kResolved) ? (v[k] & ~kResolved) : own;
+            const uint32_t nv = (v[k] & kResolved) ? (v[k] & ~kResolved) : g0 + k;
             cur = node ? nv : cur;
-            rootmask |= (node && nv == own) ? 1u << k : 0u;
-            out[k] = m ? cur : own;
+            out[k] = cur;
             mk[k >> 2] |= m << (8 * (k & 3));
         }
-        roots = __builtin_popcount(rootmask);
         // first-voxel keys start at the root block's own first voxel (global
         // roots are tile-local roots)
         if (ckey)
@@ -300,7 +301,7 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
             }
         }
     }
-    if (count) count_add(slots, roots);
+    if (count) count_add_wave(slots, roots);
 }
 
 // ---- 2. tile-face merge -----------------------------------------------------------

@@ -214,6 +214,30 @@ __device__ __forceinline__ void count_add(uint32_t *slots, uint32_t v)
     }
 }
 
+// count_add for per-lane values below 32 (a tile lane's roots: at most one per
+// block of its 16): the wave's sum by five ballots and popcounts on the scalar
+// unit, then one atomic per wave; no LDS, no barrier, and the waves of a
+// workgroup spread over the slots.  All lanes of the wave must call it.
+__device__ __forceinline__ void count_add_wave(uint32_t *slots, uint32_t v)
+{
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t bit = 0; bit < 5; ++bit) s += (uint32_t)__popcll(__ballot((v >> bit) & 1u)) << bit;
+    const uint32_t slot = (blockIdx.x * ((blockDim.x + 63) >> 6) + (threadIdx.x >> 6)) % kCountSlots;
+    if ((threadIdx.x & 63u) == 0 && s) atomicAdd(&slots[slot * kCountStride], s);
+}
+
+// Bit k of the result: bit 2k or bit 2k + 1 of s (a 2-column block holds a
+// set bit of a 32-column row mask), k < 16.
+__device__ __forceinline__ uint32_t pair_any16(uint32_t s)
+{
+    uint32_t x = (s | (s >> 1)) & 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    return (x | (x >> 8)) & 0xFFFFu;
+}
+
 // Sum the slots into *count and zero them again (called by one full wave,
 // threads 0..63, after every kernel that adds to them has finished).  Slots
 // are zero between pipeline runs: workspaces zero them when allocated and
