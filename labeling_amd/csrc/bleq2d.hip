@@ -271,7 +271,7 @@ __device__ __forceinline__ void store_block_words(uint32_t *P, size_t at, uint32
         }
         return;
     }
-    const uint32_t p13 = __shfl_up(out[13], 1, 64), p14 = __shfl_up(out[14], 1, 64), p15 = __shfl_up(out[15], 1, 64);
+    const uint32_t p13 = lane_prev(out[13]), p14 = lane_prev(out[14]), p15 = lane_prev(out[15]);
     if (full) {
         const uint32_t o = (uint32_t)(at & 3u);
         uint4 *base = reinterpret_cast<uint4 *>(dst - o);
@@ -777,7 +777,7 @@ enum : uint32_t { kModeRoot = 0, kModeRef = 1, kModeCanon = 2, kModeFirst = 3, k
 __device__ __forceinline__ void store_row_aligned(uint32_t *row, uint32_t x0, uint32_t w, uint32_t v0, uint32_t v1,
                                                   uint32_t v2, uint32_t v3, bool write, bool has_prev, bool next_in_row)
 {
-    const uint32_t p1 = __shfl_up(v1, 1, 64), p2 = __shfl_up(v2, 1, 64), p3 = __shfl_up(v3, 1, 64);
+    const uint32_t p1 = lane_prev(v1), p2 = lane_prev(v2), p3 = lane_prev(v3);
     if (!write) return;
     uint32_t *dst = row + x0;   // x0 < w
     const uint32_t o = (uint32_t)((reinterpret_cast<uintptr_t>(dst) >> 2) & 3u), lim = min(4u, w - x0);

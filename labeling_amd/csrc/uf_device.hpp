@@ -227,6 +227,15 @@ __device__ __forceinline__ void count_add_wave(uint32_t *slots, uint32_t v)
     if ((threadIdx.x & 63u) == 0 && s) atomicAdd(&slots[slot * kCountStride], s);
 }
 
+// The previous lane's v (lane 0: its own): a DPP wave shift right by one lane
+// (wave_shr:1, a GFX9-family DPP control), a VALU move with no LDS round trip.
+// Every lane of the wave must execute it.
+__device__ __forceinline__ uint32_t lane_prev(uint32_t v)
+{
+    constexpr int kWaveShr1 = 0x138;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, kWaveShr1, 0xF, 0xF, false);
+}
+
 // Bit k of the result: bit 2k or bit 2k + 1 of s (a 2-column block holds a
 // set bit of a 32-column row mask), k < 16.
 __device__ __forceinline__ uint32_t pair_any16(uint32_t s)
