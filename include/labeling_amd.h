@@ -37,8 +37,52 @@ extern "C" {
 
 typedef struct lbl_ctx lbl_ctx;
 
-/* Registry names of ALG_LIST (main.cpp:117-126). */
-typedef enum { LBL_BLEQ = 0, LBL_LBEQ = 1 } lbl_alg;
+/* Registry names of ALG_LIST (main.cpp:103-130; lbl_alg_from_name).  Every
+ * method runs on the GPU; the reference's CPU-only methods (he-run, gr-block,
+ * ocv) are computed from the same block forests with their label values.
+ *   LBL_BLEQ    "bleq"     block equivalence; 2D 8-connectivity (4 rejected), 3D 26.
+ *   LBL_LBEQ    "lbeq"     label equivalence; 2D 4 (default) or 8, 3D 26.
+ *   LBL_BIN     "bin"      binarization: label 1 on every foreground pixel, every
+ *                          mode (TBinLabeling, LabelingAlgs.cpp:23-36;
+ *                          BinLabelingKernel, LabelingAlgs.cl:18-25); 2D and 3D;
+ *                          count 1 if there is foreground, else 0.
+ *   LBL_RUNEQ   "runeq"    run equivalence (TRunEqivLabeling / TOCLRunEquivLabeling,
+ *                          LabelingAlgs.cpp:535-810,1090-1268): connectivity 4
+ *                          rejected like the reference, yet runs of adjacent
+ *                          rows join only where they share a column (IsNeib,
+ *                          LabelingAlgs.cpp:605-613), i.e. 4-connected components.
+ *                          Root mode: 1 + y * ceil(w/2) + k for the component's
+ *                          first run, the k-th (0-based) run of row y (the CPU
+ *                          values, w >> 1 run slots per row for even w);
+ *                          reference mode: 1 + y * (P2(w)/2) + k (the GPU values
+ *                          on the padded image).  2D only.
+ *   LBL_HERUN   "he-run"   He et al. run labeling (TRunLabeling, LabelingAlgs.cpp:58-205):
+ *                          4-connectivity by default, 8 on request; root and
+ *                          reference mode: the provisional label the reference
+ *                          keeps for a component, i.e. the number of runs up to
+ *                          and including its first run that have no pixel of
+ *                          the row above in reach.  Labels every pixel (the
+ *                          reference skips the last row and the last column's
+ *                          runs, LabelingAlgs.cpp:153,162-171).  2D only.
+ *   LBL_GRBLOCK "gr-block" Grana et al. block labeling (TBlockGranaLabeling,
+ *                          LabelingAlgs.cpp:47-56): 8-connectivity (4 rejected);
+ *                          root and reference mode: 1..N in the order of each
+ *                          component's first 2x2 block in block raster order
+ *                          (cvlabeling_imagelab.cpp:1523-1531).  2D only.
+ *   LBL_OCV     "ocv"      cv::connectedComponents (TOpenCVLabeling, LabelingAlgs.cpp:39-45):
+ *                          8-connectivity when 8 is asked for, else 4; every mode:
+ *                          1..N in raster order of first pixels.  2D only.
+ * LBL_LABELS_CANONICAL gives 1..N in raster order of first pixels for every
+ * method except bin. */
+typedef enum {
+    LBL_BLEQ = 0,
+    LBL_LBEQ = 1,
+    LBL_BIN = 2,
+    LBL_RUNEQ = 3,
+    LBL_HERUN = 4,
+    LBL_GRBLOCK = 5,
+    LBL_OCV = 6
+} lbl_alg;
 
 typedef enum {
     LBL_OK = 0,
@@ -123,9 +167,9 @@ lbl_status lbl_set_label_mode(lbl_ctx *ctx, lbl_label_mode mode);
 lbl_status lbl_set_iteration_cap(lbl_ctx *ctx, uint32_t cap);
 
 /* Registry lookup by the reference's algorithm names, mirroring
- * SetLabelingAlg (main.cpp:373-395): "bleq" or "lbeq".  Returns LBL_EUNSUPPORTED
- * for names of algorithms this library does not provide ("bin", "runeq",
- * "he-run", "gr-block", "ocv") and LBL_EINVAL for unknown names. */
+ * SetLabelingAlg (main.cpp:373-395): "bleq", "lbeq", "bin", "runeq", "he-run",
+ * "gr-block" or "ocv" (every ALG_LIST entry, main.cpp:103-130).  Returns
+ * LBL_EINVAL for unknown names. */
 lbl_status lbl_alg_from_name(const char *name, lbl_alg *out);
 
 /* ---- single image, host buffers ------------------------------------------ */
@@ -145,7 +189,9 @@ lbl_status lbl_label2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *px, uint32_t h,
 /* 3D labeling of a host volume.  Replaces IOCLLabeling3D::Label
  * (LabelingTools.hpp:154, LabelingTools.cpp:180-221) running
  * TOCLBlockEquivalence3D::DoOCLLabel3D (LabelingAlgs.cpp:1520-1536) for bleq or
- * TOCLLabelEquivalence3D::DoOCLLabel3D (LabelingAlgs.cpp:1362-1396) for lbeq.
+ * TOCLLabelEquivalence3D::DoOCLLabel3D (LabelingAlgs.cpp:1362-1396) for lbeq,
+ * TOCLBinLabeling3D (LabelingAlgs.cpp:1270-1320) for bin; the registry has no
+ * 3D variant of the other methods (LBL_EUNSUPPORTED).
  * labels: d*h*w dense (cropped). */
 lbl_status lbl_label3d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *vox, uint32_t d, uint32_t h,
                        uint32_t w, int conn, uint32_t *labels, uint32_t *n_components,
@@ -175,7 +221,8 @@ lbl_status lbl_label3d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_vox, uint
  * need more per pixel).  Every call on a device shares that array
  * and clears what must start at zero on its own stream, so calls that overlap
  * on one device race: serialise them (one stream, or events).  Labels, count
- * and errors are those of lbl_label2d_dev. */
+ * and errors are those of lbl_label2d_dev.  bleq and lbeq only (other methods:
+ * LBL_EUNSUPPORTED). */
 lbl_status lbl_label2d_dev_noalloc(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d_px, uint32_t h, uint32_t w,
                                    size_t pitch_bytes, int conn, uint32_t *d_labels, uint32_t *d_ncomp,
                                    void *stream);

@@ -26,9 +26,9 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "labeling_amd.h")
 # lbl_status
 OK, EINVAL, EUNSUPPORTED, ENOMEM, EDEVICE, EITER, EBUSY = range(7)
 STATUS_NAMES = ["LBL_OK", "LBL_EINVAL", "LBL_EUNSUPPORTED", "LBL_ENOMEM", "LBL_EDEVICE", "LBL_EITER", "LBL_EBUSY"]
-# lbl_alg
-BLEQ, LBEQ = 0, 1
-ALGS = {"bleq": BLEQ, "lbeq": LBEQ}
+# lbl_alg: every ALG_LIST entry of the reference (main.cpp:103-130)
+BLEQ, LBEQ, BIN, RUNEQ, HERUN, GRBLOCK, OCV = range(7)
+ALGS = {"bleq": BLEQ, "lbeq": LBEQ, "bin": BIN, "runeq": RUNEQ, "he-run": HERUN, "gr-block": GRBLOCK, "ocv": OCV}
 # lbl_label_mode
 LABELS_ROOT, LABELS_REFERENCE, LABELS_CANONICAL, LABELS_REFERENCE_UNCROPPED = 0, 1, 2, 3
 
@@ -182,8 +182,7 @@ def synth_u64(seed: int, stream: int, i: int) -> int:
 def _alg_id(alg) -> int:
     if isinstance(alg, str):
         if alg not in ALGS:
-            raise LabelingError(EUNSUPPORTED if alg in ("bin", "runeq", "he-run", "gr-block", "ocv") else EINVAL,
-                                f"algorithm {alg!r} is not provided")
+            raise LabelingError(EINVAL, f"unknown algorithm {alg!r}")
         return ALGS[alg]
     return int(alg)
 

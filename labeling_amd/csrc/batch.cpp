@@ -249,7 +249,8 @@ extern "C" lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
     *out = nullptr;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     if (!images || n == 0) return bfail(ctx, LBL_EINVAL, "lbl_batch_create: empty batch");
-    if (alg == LBL_LBEQ) return bfail(ctx, LBL_EUNSUPPORTED, "lbl_batch_create: batches run bleq only");
+    if ((int)alg > (int)LBL_BLEQ && (int)alg <= (int)LBL_OCV)
+        return bfail(ctx, LBL_EUNSUPPORTED, "lbl_batch_create: batches run bleq only");
     if (alg != LBL_BLEQ) return bfail(ctx, LBL_EINVAL, "unknown algorithm id");
     if (conn == 4) return bfail(ctx, LBL_EUNSUPPORTED, "bleq: method does not support 4-connectivity");
     if (conn != 0 && conn != 8) return bfail(ctx, LBL_EINVAL, "connectivity must be 0, 4 or 8 for 2D");

@@ -37,6 +37,7 @@
 //
 // HBM traffic per pixel: 1 B image in + 1 B block words out in (1), 1 B block
 // words in + 4 B labels out in (4), plus the border blocks in (2)-(3).
+#include <algorithm>
 #include <type_traits>
 
 #include "bleq2d.hpp"
@@ -762,7 +763,47 @@ __device__ __forceinline__ void lane_roots(const Geom2D &G, const uint32_t v[FIN
 // Label values the final kernel writes: fixed per launch (MODE), or per image
 // from its flags (kModeAny, batches).
 // kModeFirst: 1 + the component's first pixel index (the label equivalence values).
-enum : uint32_t { kModeRoot = 0, kModeRef = 1, kModeCanon = 2, kModeFirst = 3, kModeAny = 4 };
+// kModeBlockRank: 1..N in the order of the components' root blocks (Grana's
+//   block-based labeling numbers its components so, cvlabeling_imagelab.cpp:1523-1531).
+// kModeRunRank: the number of runs up to and including the component's first
+//   run that have no pixel of the row above in reach (the provisional label He
+//   et al.'s run labeling keeps for the component, LabelingAlgs.cpp:80-130);
+//   the run bits hold those runs' first pixels.
+// kModeRunIndex: 1 + first-run row * run_w2 + the index of the first run in its
+//   row (the run equivalence values, LabelingAlgs.cpp:573-588,611-691); the run
+//   bits hold every run's first pixel.
+enum : uint32_t {
+    kModeRoot = 0, kModeRef = 1, kModeCanon = 2, kModeFirst = 3, kModeAny = 4,
+    kModeBlockRank = 5, kModeRunRank = 6, kModeRunIndex = 7
+};
+
+// The label of a component of 2x2 blocks with root block r, for the label
+// values MODE (kModeAny: from the image's flags): the reference's raw values
+// are 1 + the root's block index in its padded grid (LabelingAlgs.cl:202,317);
+// the label equivalence values 1 + the first pixel's index, or with kRefLabels
+// the reference's 0-based index in its padded image (DistrInitKernel,
+// LabelingAlgs.cl:37).
+template <uint32_t MODE>
+__device__ __forceinline__ uint32_t root_value(const Geom2D &G, uint32_t r)
+{
+    const bool ref = MODE == kModeRef || (MODE == kModeAny && (G.flags & kRefLabels));
+    const bool canon = MODE == kModeCanon || (MODE == kModeAny && (G.flags & kCanonLabels));
+    if (canon) return canon_label(G.canon, G.canon.key[r]);
+    if (MODE == kModeBlockRank) return canon_label(G.canon, r);
+    if (MODE == kModeRunRank) {   // the first pixel's key, in the run bits' row-padded layout
+        const uint32_t k = G.canon.key[r], y = k / G.w;
+        return canon_label(G.canon, y * G.w64 + (k - y * G.w));
+    }
+    if (MODE == kModeFirst) {
+        const uint32_t k = G.canon.key[r];
+        return (G.flags & kRefLabels) ? (k / G.w) * (2 * G.ref_bw) + k % G.w : k + 1u;
+    }
+    if (ref) {
+        const uint32_t ry = div_by(r, G.bdiv);
+        return 1u + (r - ry * G.bw) + ry * G.ref_bw;
+    }
+    return r + 1u;
+}
 
 // Store a lane's four labels v0..v3 of one pixel row, columns x0 .. x0 + 3
 // (those < w), row starting at `row`.  A row whose lanes' labels are not
@@ -811,7 +852,6 @@ __device__ __forceinline__ void final_span_labels(const Geom2D &G, const Span &s
 #pragma unroll
     for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) lab[k] = 0;
     if (sp.n) {
-        const bool ref = MODE == kModeRef || (MODE == kModeAny && (G.flags & kRefLabels));
         const bool canon = MODE == kModeCanon || (MODE == kModeAny && (G.flags & kCanonLabels));
         uint32_t prev_root = ~0u, prev_lab = 0;
 #pragma unroll
@@ -821,26 +861,21 @@ __device__ __forceinline__ void final_span_labels(const Geom2D &G, const Span &s
             const uint32_t r = root[k];
             if (r != prev_root) {
                 prev_root = r;
-                // reference raw values: 1 + block index in the padded grid (LabelingAlgs.cl:202,317);
-                // label equivalence values: 1 + the first pixel's index, or with
-                // kRefLabels the reference's 0-based index in its padded image
+                // label equivalence values (CONN4): 1 + the first pixel's index, or
+                // with kRefLabels the reference's 0-based index in its padded image
                 // (width 2 ref_bw; DistrInitKernel, LabelingAlgs.cl:37)
                 if (CONN4) {   // the root block (1x2, the component's first) holds its first pixel
                     const uint32_t y = div_by(r, G.bdiv), x = 2 * (r - y * bw) + ((G.P[r] & 1u) ? 0u : 1u);
-                    prev_lab = canon ? canon_label(G.canon, r)
-                                     : ((G.flags & kRefLabels) ? y * (2 * G.ref_bw) + x : 1u + y * w + x);
-                } else if (canon) {
-                    prev_lab = canon_label(G.canon, G.canon.key[r]);
-                } else if (MODE == kModeFirst) {
-                    const uint32_t k = G.canon.key[r];
-                    prev_lab = (G.flags & kRefLabels) ? (k / w) * (2 * G.ref_bw) + k % w : k + 1u;
+                    if (MODE == kModeRunRank)
+                        prev_lab = canon_label(G.canon, y * G.w64 + x);
+                    else if (MODE == kModeRunIndex)   // runs of row y before x: the bits between the row's start and x
+                        prev_lab = 1u + y * G.run_w2 + canon_label(G.canon, y * G.w64 + x) -
+                                   canon_label(G.canon, y * G.w64);
+                    else
+                        prev_lab = canon ? canon_label(G.canon, r)
+                                         : ((G.flags & kRefLabels) ? y * (2 * G.ref_bw) + x : 1u + y * w + x);
                 } else {
-                    if (ref) {
-                        const uint32_t ry = div_by(r, G.bdiv);
-                        prev_lab = 1u + (r - ry * bw) + ry * G.ref_bw;
-                    } else {
-                        prev_lab = r + 1u;
-                    }
+                    prev_lab = root_value<MODE>(G, r);
                 }
             }
             lab[k] = prev_lab;
@@ -1041,6 +1076,89 @@ __global__ __launch_bounds__(kCanonThreads) void bleq2d_canon_prefix(Src src)
     canon_prefix_chunk(G.canon, lc);
 }
 
+// ---- run bits: the He et al. and run equivalence values ----------------------------------
+//
+// One wave per pixel row y sets bit x of 64-bit word y * w64 / 64 + x / 64 of
+// the canonical bit array (rows padded to whole words, so every word belongs
+// to one lane and no atomics are needed) where a run of row y starts at column
+// x.  LC: only the runs with no pixel of row y - 1 in columns start - conpix ..
+// end + conpix, the runs for which He et al.'s run labeling opens a new
+// provisional label (TRunLabeling::SetRunLabel, LabelingAlgs.cpp:80-130).  The
+// canonical scan over these bits then ranks a component's first run among them.
+//
+// "Touched" (a pixel of the run has a pixel of the row above in reach) is found
+// bit-parallel: contact bits are filled towards bit 0 through each run's set
+// bits by log-step shifts, and a run that continues into the next words takes
+// their contacts by ballots over the wave (lanes = consecutive words); the
+// row's words are walked in chunks of 64 from the right, the touched state of
+// a run entering a chunk from its right carried over.
+
+__device__ __forceinline__ unsigned long long row_bits64(const uint8_t *row, uint32_t x0, uint32_t w, bool vec)
+{
+    const unsigned long long lo = row_bits32(row, x0, w, vec);
+    const unsigned long long hi = x0 + 32 < w ? row_bits32(row, x0 + 32, w, vec) : 0ull;
+    return lo | (hi << 32);
+}
+
+template <bool LC>
+__global__ __launch_bounds__(64) void runs2d_bits(Geom2D G, uint32_t conpix)
+{
+    const uint32_t y = blockIdx.x, lane = threadIdx.x, w = G.w, nw = G.w64 / 64;
+    const bool vec = (G.flags & kVecIn) != 0;
+    const uint8_t *row = G.px + (size_t)y * G.pitch;
+    const uint8_t *up = row - G.pitch;   // read only when y > 0
+    unsigned long long *out = G.canon.bits + (size_t)y * nw;
+    bool carry = false, carry_b0 = false;   // the run entering the current chunk from its right: touched / exists
+    for (uint32_t c = (nw + 63) / 64; c-- > 0;) {
+        const uint32_t j = c * 64 + lane, x0 = 64 * j;
+        unsigned long long r = 0, u = 0;
+        uint32_t prev = 0, ul = 0, ur = 0;
+        if (j < nw) {
+            r = row_bits64(row, x0, w, vec);
+            prev = x0 > 0 && row[x0 - 1] != 0;
+            if (LC && y > 0) {
+                u = row_bits64(up, x0, w, vec);
+                ul = x0 > 0 && up[x0 - 1] != 0;
+                ur = x0 + 64 < w && up[x0 + 64] != 0;
+            }
+        }
+        const unsigned long long starts = r & ~((r << 1) | prev);
+        unsigned long long res = starts;
+        if (LC) {
+            unsigned long long reach = u;
+            if (conpix) reach |= (u << 1) | ul | (u >> 1) | ((unsigned long long)ur << 63);
+            // m: bits of r with a contact at or above them in their run (within the word)
+            unsigned long long m = r & reach, p = r;
+#pragma unroll
+            for (uint32_t sh = 1; sh < 64; sh <<= 1) {
+                m |= (m >> sh) & p;
+                p &= p >> sh;
+            }
+            const bool full = r == ~0ull, b0 = (r & 1ull) != 0, m0 = (m & 1ull) != 0;
+            const unsigned long long F = __ballot(full), M = __ballot(m0), B0 = __ballot(b0);
+            const unsigned long long above = lane < 63 ? ~0ull << (lane + 1) : 0ull;
+            const bool next_b0 = lane < 63 ? ((B0 >> (lane + 1)) & 1ull) != 0 : carry_b0;
+            // the run holding bit 63 continues into the next word: its part there,
+            // up to the first word that is not all foreground, decides
+            bool t_in = false;
+            if ((r >> 63) && next_b0) {
+                const unsigned long long nf = ~F & above;
+                const uint32_t k = nf ? (uint32_t)__builtin_ctzll(nf) : 64u;
+                const unsigned long long span = (k < 64 ? ((2ull << k) - 1ull) : ~0ull) & above;
+                t_in = (M & span) != 0 || (k == 64 && carry);
+            }
+            // the run holding bit 63 starts at the highest start bit, if it starts here
+            const unsigned long long top = (r >> 63) && starts ? 1ull << (63 - __builtin_clzll(starts)) : 0ull;
+            res = starts & ~m & ~(t_in ? top : 0ull);
+            // for the chunk to the left: lane 0's run at bit 0, touched (with its continuation) / exists
+            const bool bottom = m0 || (full && t_in);
+            carry = __shfl((int)bottom, 0, 64) != 0;
+            carry_b0 = __shfl((int)b0, 0, 64) != 0;
+        }
+        if (j < nw) out[j] = res;
+    }
+}
+
 // ---- host side --------------------------------------------------------------------
 
 template <uint32_t TBW, uint32_t TBH>
@@ -1056,6 +1174,19 @@ struct Variant {
         case kModeRef: hipLaunchKernelGGL((bleq2d_final<Src, kModeRef, CONN4, EXACT>), grid, block, 0, s, src); break;
         case kModeCanon: hipLaunchKernelGGL((bleq2d_final<Src, kModeCanon, CONN4, EXACT>), grid, block, 0, s, src); break;
         case kModeFirst: hipLaunchKernelGGL((bleq2d_final<Src, kModeFirst, CONN4, EXACT>), grid, block, 0, s, src); break;
+        // the values of the other registry methods: single images only
+        case kModeBlockRank:
+            if constexpr (Src::kSingle && !CONN4)
+                hipLaunchKernelGGL((bleq2d_final<Src, kModeBlockRank, CONN4, EXACT>), grid, block, 0, s, src);
+            break;
+        case kModeRunRank:
+            if constexpr (Src::kSingle)
+                hipLaunchKernelGGL((bleq2d_final<Src, kModeRunRank, CONN4, EXACT>), grid, block, 0, s, src);
+            break;
+        case kModeRunIndex:
+            if constexpr (Src::kSingle && CONN4)
+                hipLaunchKernelGGL((bleq2d_final<Src, kModeRunIndex, CONN4, EXACT>), grid, block, 0, s, src);
+            break;
         default: hipLaunchKernelGGL((bleq2d_final<Src, kModeAny, CONN4, EXACT>), grid, block, 0, s, src); break;
         }
     }
@@ -1064,7 +1195,12 @@ struct Variant {
     static hipError_t launch(const Src &src, uint32_t total_tiles, uint32_t total_final, hipStream_t s,
                              KernelTimer *kt, bool flatten, uint32_t canon_chunks, uint32_t mode)
     {
-        const bool canon = canon_chunks != 0;   // some image asks for canonical labels
+        // run bits (he-run / runeq values) and block-order ranks (gr-block) use
+        // the canonical scan over their own bits; canon: first-pixel canonical
+        // numbering (some image asks for canonical labels)
+        const bool run_bits = mode == kModeRunRank || mode == kModeRunIndex;
+        const bool block_rank = mode == kModeBlockRank;
+        const bool canon = canon_chunks != 0 && !run_bits && !block_rank;
         const uint32_t merge_grid =
             (uint32_t)(((uint64_t)total_tiles * (TBW / LANE_BLOCKS + TBH) + merge_jobs<Src>() - 1) / merge_jobs<Src>());
         const uint32_t border_grid = (uint32_t)(((uint64_t)total_tiles * Border<TBW, TBH>::JOBS + 255) / 256);
@@ -1086,7 +1222,7 @@ struct Variant {
             hipLaunchKernelGGL((bleq2d_flatten<TBW, TBH, Src>), dim3(border_grid), dim3(256), 0, s, src, total_tiles);
             if (kt) kt->mark(s, "bleq2d_flatten");
         }
-        if (!CONN4 && (mode == kModeFirst || canon)) {
+        if (!CONN4 && (mode == kModeFirst || mode == kModeRunRank || canon)) {
             hipLaunchKernelGGL((bleq2d_keys<Src>), dim3(total_final), dim3(NT_FINAL), 0, s, src);   // clears the bits
             if (kt) kt->mark(s, "bleq2d_keys");
         }
@@ -1097,6 +1233,25 @@ struct Variant {
             }
             hipLaunchKernelGGL((bleq2d_mark<Src, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_mark");
+            hipLaunchKernelGGL((bleq2d_canon_partial<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
+            if (kt) kt->mark(s, "canon_partial");
+            hipLaunchKernelGGL((bleq2d_canon_prefix<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
+            if (kt) kt->mark(s, "canon_prefix");
+        }
+        if (block_rank || run_bits) {
+            if (block_rank) {   // one bit per root, at the root block's own index
+                hipLaunchKernelGGL((bleq2d_canon_clear<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
+                if (kt) kt->mark(s, "canon_clear");
+                hipLaunchKernelGGL((bleq2d_mark<Src, true>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
+                if (kt) kt->mark(s, "bleq2d_mark");
+            } else if constexpr (Src::kSingle) {   // every bit word is written
+                const uint32_t conpix = CONN4 ? 0u : 1u;   // he-run 8-connectivity runs on 2x2 blocks
+                if (mode == kModeRunRank)
+                    hipLaunchKernelGGL(runs2d_bits<true>, dim3(src.g.h), dim3(64), 0, s, src.g, conpix);
+                else
+                    hipLaunchKernelGGL(runs2d_bits<false>, dim3(src.g.h), dim3(64), 0, s, src.g, conpix);
+                if (kt) kt->mark(s, "runs2d_bits");
+            }
             hipLaunchKernelGGL((bleq2d_canon_partial<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
             if (kt) kt->mark(s, "canon_partial");
             hipLaunchKernelGGL((bleq2d_canon_prefix<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
@@ -1232,6 +1387,95 @@ hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hip
     if (kt) kt->begin(s);
     return Variant<TBW, TBH>::launch<One, true>(one, g.ntiles, g.nfinal, s, kt, false,
                                                 canon ? canon_chunks(g.canon.nwords) : 0u, canon ? kModeCanon : kModeFirst);
+}
+
+// ---- the other registry methods' label values, on the same block forests ------------
+
+size_t grblock2d_workspace(uint32_t h, uint32_t w) { return bleq2d_workspace(h, w, true); }
+
+// Grana's block-based labeling ("gr-block", TBlockGranaLabeling, LabelingAlgs.cpp:47-56):
+// 8-connectivity, components numbered 1..N in the order of their first 2x2
+// block (the root block): a bit per root at its block index, ranked by the
+// canonical scan.  LBL_LABELS_CANONICAL: first-pixel order instead.
+hipError_t grblock2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt)
+{
+    if (opt.mode == LBL_LABELS_CANONICAL) return bleq2d_run(im, opt, ws, s, kt);
+    One one;
+    Geom2D &g = one.g;
+    g = Geom2D{};
+    g.px = im.px;
+    g.pitch = im.pitch;
+    g.h = im.h;
+    g.w = im.w;
+    g.labels = im.labels;
+    g.count = im.count;
+    g.slots = static_cast<uint32_t *>(ws);
+    g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
+    bleq2d_geometry(g, opt.tile2d);
+    g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256((size_t)g.bw * g.bh * sizeof(uint32_t)),
+                           (uint64_t)g.bw * g.bh, 0);
+    if (kt) kt->begin(s);
+    return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, canon_chunks(g.canon.nwords),
+                    kModeBlockRank);
+}
+
+size_t runs2d_workspace(uint32_t h, uint32_t w, bool conn8, bool canonical)
+{
+    const uint64_t bw = (w + 1ull) / 2, w64 = (w + 63ull) / 64 * 64;
+    const uint64_t nodes = conn8 ? bw * ((h + 1ull) / 2) : bw * h;
+    const size_t own = align256(kCountBytes) + align256(nodes * sizeof(uint32_t)) +
+                       canon_workspace((uint64_t)h * w64, conn8 ? nodes : 0);
+    if (!canonical) return own;
+    return std::max(own, conn8 ? bleq2d_workspace(h, w, true) : bleq2d_conn4_workspace(h, w, true));
+}
+
+// He et al.'s run labeling ("he-run", TRunLabeling, LabelingAlgs.cpp:58-205;
+// he = true) and the run equivalence labeling ("runeq", TRunEqivLabeling /
+// TOCLRunEquivLabeling, LabelingAlgs.cpp:535-810,1090-1268; he = false).
+// Components: he-run 4-connectivity on 1x2 blocks, 8 on 2x2 blocks with
+// first-pixel keys; runeq joins runs of adjacent rows that share a column
+// (IsNeib, LabelingAlgs.cpp:605-613), which is 4-connectivity.  Values from the
+// run bits (runs2d_bits): he-run the rank of the component's first run among
+// the runs that open a provisional label; runeq 1 + row * run_w2 + the index
+// of the first run in its row, run_w2 = ceil(w / 2) (the reference CPU's
+// w >> 1 run slots per row, for even w) or, with LBL_LABELS_REFERENCE, half the
+// reference's padded width (the GPU run slots, LabelingAlgs.cpp:1253).
+// LBL_LABELS_CANONICAL: first-pixel order on the same forests.
+hipError_t runs2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt, bool he,
+                      int conn)
+{
+    const bool conn8 = he && conn == 8;
+    if (opt.mode == LBL_LABELS_CANONICAL)
+        return conn8 ? bleq2d_run(im, opt, ws, s, kt) : bleq2d_run_conn4(im, opt, ws, s, kt);
+    One one;
+    Geom2D &g = one.g;
+    g = Geom2D{};
+    g.px = im.px;
+    g.pitch = im.pitch;
+    g.h = im.h;
+    g.w = im.w;
+    g.labels = im.labels;
+    g.count = im.count;
+    g.slots = static_cast<uint32_t *>(ws);
+    g.P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
+    g.flags = conn8 ? kKeys | kPixelKeys : 0u;
+    const uint32_t tile = conn8 ? opt.tile2d : 2u;   // 1x2 blocks: the label equivalence tile (below)
+    bleq2d_geometry(g, tile);
+    if (!conn8) {   // 1x2 blocks, one pixel row per block row (as bleq2d_run_conn4)
+        g.bh = g.h;
+        g.tiles_x = (g.bw + 128 - 1) / 128;
+        g.ntiles = g.tiles_x * ((g.bh + 64 - 1) / 64);
+        g.nfinal = (uint32_t)(((uint64_t)g.bw * g.bh + NT_FINAL * FINAL_BLOCKS - 1) / (NT_FINAL * FINAL_BLOCKS));
+    }
+    g.w64 = (g.w + 63u) / 64 * 64;
+    g.run_w2 = opt.mode == LBL_LABELS_REFERENCE ? (uint32_t)(ref_pad2(g.w) / 2) : (g.w + 1u) / 2;
+    const uint64_t nodes = (uint64_t)g.bw * g.bh;
+    g.canon = canon_layout(reinterpret_cast<uint8_t *>(g.P) + align256(nodes * sizeof(uint32_t)),
+                           (uint64_t)g.h * g.w64, conn8 ? nodes : 0);
+    const uint32_t mode = he ? kModeRunRank : kModeRunIndex, chunks = canon_chunks(g.canon.nwords);
+    if (kt) kt->begin(s);
+    if (conn8) return dispatch(opt.tile2d, one, g.ntiles, g.nfinal, s, kt, opt.flatten, chunks, mode);
+    return Variant<128, 64>::launch<One, true>(one, g.ntiles, g.nfinal, s, kt, false, chunks, mode);
 }
 
 hipError_t bleq2d_run_batch(const Geom2D *d_geoms, uint32_t n, const BatchTables &tables, uint32_t total_tiles,

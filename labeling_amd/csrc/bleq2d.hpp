@@ -35,6 +35,8 @@ struct Geom2D {
     uint32_t chunk_base;        // first global canonical scan chunk id (batch)
     uint32_t flags;
     uint32_t ref_bw;            // block-grid width of the reference's padded image
+    uint32_t w64;               // run-bit key row width: w rounded up to 64 (he-run / runeq values)
+    uint32_t run_w2;            // run numbers per row of the runeq values
     Canon canon;                // canonical numbering state (kCanon)
 };
 

@@ -79,25 +79,44 @@ lbl_status ensure(lbl_ctx *ctx, void **buf, size_t *cap, size_t need, bool *fres
 // Connectivity defaults and support, as in the reference:
 // bleq 2D: 8 (COH_4 rejected, LabelingAlgs.cpp:338,1070); lbeq 2D: default 4
 // (LabelingAlgs.cpp:213), 8 on request; 3D: default only = 26 (LabelingTools.cpp:185).
+// gr-block and runeq reject 4 (LabelingAlgs.cpp:51,537,1242; runeq then joins
+// runs by shared columns, its own rule); he-run and ocv use 8 only when asked
+// (LabelingAlgs.cpp:42,140); bin ignores it.
 lbl_status normalize_conn2d(lbl_ctx *ctx, lbl_alg alg, int conn, int *out)
 {
-    if (alg == LBL_BLEQ) {
-        if (conn == 0 || conn == 8) { *out = 8; return LBL_OK; }
-        if (conn == 4) return fail(ctx, LBL_EUNSUPPORTED, "bleq: method does not support 4-connectivity");
-    } else if (alg == LBL_LBEQ) {
-        if (conn == 0 || conn == 4) { *out = 4; return LBL_OK; }
-        if (conn == 8) { *out = 8; return LBL_OK; }
-    } else {
-        return fail(ctx, LBL_EINVAL, "unknown algorithm id " + std::to_string((int)alg));
+    if (conn != 0 && conn != 4 && conn != 8)
+        return fail(ctx, LBL_EINVAL, "connectivity must be 0, 4 or 8 for 2D, got " + std::to_string(conn));
+    switch (alg) {
+    case LBL_BLEQ:
+    case LBL_GRBLOCK:
+    case LBL_RUNEQ:
+        if (conn == 4)
+            return fail(ctx, LBL_EUNSUPPORTED, std::string(alg == LBL_BLEQ ? "bleq" : alg == LBL_GRBLOCK ? "gr-block" : "runeq") +
+                                                   ": method does not support 4-connectivity");
+        *out = 8;
+        return LBL_OK;
+    case LBL_LBEQ:
+    case LBL_HERUN:
+    case LBL_OCV: *out = conn == 8 ? 8 : 4; return LBL_OK;
+    case LBL_BIN: *out = conn == 0 ? 8 : conn; return LBL_OK;
     }
-    return fail(ctx, LBL_EINVAL, "connectivity must be 0, 4 or 8 for 2D, got " + std::to_string(conn));
+    return fail(ctx, LBL_EINVAL, "unknown algorithm id " + std::to_string((int)alg));
 }
 
 lbl_status normalize_conn3d(lbl_ctx *ctx, lbl_alg alg, int conn)
 {
-    if (alg != LBL_BLEQ && alg != LBL_LBEQ) return fail(ctx, LBL_EINVAL, "unknown algorithm id " + std::to_string((int)alg));
+    if ((int)alg < (int)LBL_BLEQ || (int)alg > (int)LBL_OCV)
+        return fail(ctx, LBL_EINVAL, "unknown algorithm id " + std::to_string((int)alg));
+    if (alg != LBL_BLEQ && alg != LBL_LBEQ && alg != LBL_BIN)   // the registry's 3D entries (main.cpp:103-130)
+        return fail(ctx, LBL_EUNSUPPORTED, "this method has no 3D variant");
     if (conn == 0 || conn == 26) return LBL_OK;
     return fail(ctx, LBL_EUNSUPPORTED, "3D labeling supports only the default 26-connectivity");
+}
+
+// The forest a 2D method runs on: 2x2 blocks (8-connectivity) or 1x2 blocks (4).
+bool on_2x2_blocks(lbl_alg alg, int conn)
+{
+    return alg == LBL_BLEQ || alg == LBL_GRBLOCK || ((alg == LBL_HERUN || alg == LBL_OCV || alg == LBL_LBEQ) && conn == 8);
 }
 
 lbl_status check_mode(lbl_ctx *ctx, lbl_alg alg, bool is3d)
@@ -117,13 +136,19 @@ uint64_t labels3d_elems(lbl_label_mode mode, uint32_t d, uint32_t h, uint32_t w)
     return (uint64_t)ref_pad3(d) * ref_pad3(h) * ref_pad3(w);
 }
 
-lbl_status check_size2d(lbl_ctx *ctx, lbl_alg alg, uint32_t h, uint32_t w, size_t pitch)
+lbl_status check_size2d(lbl_ctx *ctx, lbl_alg alg, int conn, uint32_t h, uint32_t w, size_t pitch)
 {
     if (h == 0 || w == 0) return fail(ctx, LBL_EINVAL, "input image is empty");
     if (pitch < w) return fail(ctx, LBL_EINVAL, "pitch is smaller than the width");
     if ((uint64_t)h * w > kMaxElems) return fail(ctx, LBL_EUNSUPPORTED, "image has more than 2^32-2 pixels");
-    if (alg == LBL_BLEQ && ((h + 1ull) / 2) * ((w + 1ull) / 2) > kMaxBlocks2D)
-        return fail(ctx, LBL_EUNSUPPORTED, "bleq 2D supports at most 2^28 2x2 blocks");
+    // lbeq falls back to its sweeps beyond the block forests' limit; the others run on the forests only
+    if (alg != LBL_LBEQ && alg != LBL_BIN) {
+        const uint64_t blocks = on_2x2_blocks(alg, conn) ? ((h + 1ull) / 2) * ((w + 1ull) / 2) : h * ((w + 1ull) / 2);
+        if (blocks > kMaxBlocks2D)
+            return fail(ctx, LBL_EUNSUPPORTED, "this method supports at most 2^28 blocks (2x2, or 1x2 with 4-connectivity)");
+    }
+    if ((alg == LBL_RUNEQ || alg == LBL_HERUN) && (uint64_t)h * ((w + 63ull) / 64 * 64) > kMaxElems)
+        return fail(ctx, LBL_EUNSUPPORTED, "run numbering needs h * (w rounded up to 64) below 2^32");
     // reference values index the reference's padded image (or its block grid)
     if (ctx->opt.mode == LBL_LABELS_REFERENCE && ref_pad2(h) * ref_pad2(w) > kMaxElems)
         return fail(ctx, LBL_EUNSUPPORTED, "reference label values need a padded image of at most 2^32-2 pixels");
@@ -141,15 +166,24 @@ lbl_status check_size3d(lbl_ctx *ctx, uint32_t d, uint32_t h, uint32_t w)
     return LBL_OK;
 }
 
-size_t workspace2d(lbl_alg alg, uint32_t h, uint32_t w, lbl_label_mode mode)
+size_t workspace2d(lbl_alg alg, int conn, uint32_t h, uint32_t w, lbl_label_mode mode)
 {
     const bool canon = mode == LBL_LABELS_CANONICAL;
-    return alg == LBL_BLEQ ? bleq2d_workspace(h, w, canon) : lbeq2d_workspace(h, w, canon);
+    switch (alg) {
+    case LBL_BLEQ: return bleq2d_workspace(h, w, canon);
+    case LBL_GRBLOCK: return grblock2d_workspace(h, w);
+    case LBL_RUNEQ: return runs2d_workspace(h, w, false, canon);
+    case LBL_HERUN: return runs2d_workspace(h, w, conn == 8, canon);
+    case LBL_OCV: return conn == 8 ? bleq2d_workspace(h, w, true) : bleq2d_conn4_workspace(h, w, true);
+    case LBL_BIN: return align256(kCountBytes);
+    default: return lbeq2d_workspace(h, w, canon);
+    }
 }
 
 size_t workspace3d(lbl_alg alg, uint32_t d, uint32_t h, uint32_t w, lbl_label_mode mode)
 {
     const bool canon = mode == LBL_LABELS_CANONICAL;
+    if (alg == LBL_BIN) return align256(kCountBytes);
     return alg == LBL_BLEQ ? bleq3d_workspace(d, h, w, canon) : lbeq3d_workspace(d, h, w, canon);
 }
 
@@ -278,11 +312,16 @@ extern "C" lbl_status lbl_set_iteration_cap(lbl_ctx *ctx, uint32_t cap)
 extern "C" lbl_status lbl_alg_from_name(const char *name, lbl_alg *out)
 {
     if (!name || !out) return LBL_EINVAL;
-    if (!std::strcmp(name, "bleq")) { *out = LBL_BLEQ; return LBL_OK; }
-    if (!std::strcmp(name, "lbeq")) { *out = LBL_LBEQ; return LBL_OK; }
-    static const char *other[] = {"bin", "runeq", "he-run", "gr-block", "ocv"};   // main.cpp:103-130
-    for (const char *o : other)
-        if (!std::strcmp(name, o)) return LBL_EUNSUPPORTED;
+    static const struct {
+        const char *name;
+        lbl_alg alg;
+    } list[] = {{"bleq", LBL_BLEQ}, {"lbeq", LBL_LBEQ}, {"bin", LBL_BIN}, {"runeq", LBL_RUNEQ},
+                {"he-run", LBL_HERUN}, {"gr-block", LBL_GRBLOCK}, {"ocv", LBL_OCV}};   // main.cpp:103-130
+    for (const auto &e : list)
+        if (!std::strcmp(name, e.name)) {
+            *out = e.alg;
+            return LBL_OK;
+        }
     return LBL_EINVAL;
 }
 
@@ -331,15 +370,28 @@ static lbl_status launch2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &i
         opt.tile2d = it->second.tile;
         opt.flatten = it->second.flatten;
     }
-    hipError_t e = alg == LBL_BLEQ ? bleq2d_run(im, opt, ws, s, kt) : lbeq2d_run(im, conn, opt, ws, s, kt);
-    if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq2d launch" : "lbeq2d launch", e);
+    hipError_t e = hipSuccess;
+    switch (alg) {
+    case LBL_BLEQ: e = bleq2d_run(im, opt, ws, s, kt); break;
+    case LBL_LBEQ: e = lbeq2d_run(im, conn, opt, ws, s, kt); break;
+    case LBL_GRBLOCK: e = grblock2d_run(im, opt, ws, s, kt); break;
+    case LBL_RUNEQ: e = runs2d_run(im, opt, ws, s, kt, false, conn); break;
+    case LBL_HERUN: e = runs2d_run(im, opt, ws, s, kt, true, conn); break;
+    case LBL_OCV: {   // 1..N in raster order of first pixels, whatever the mode
+        opt.mode = LBL_LABELS_CANONICAL;
+        e = conn == 8 ? bleq2d_run(im, opt, ws, s, kt) : bleq2d_run_conn4(im, opt, ws, s, kt);
+        break;
+    }
+    case LBL_BIN: e = bin2d_run(im, ws, s, kt); break;
+    }
+    if (e != hipSuccess) return hip_fail(ctx, "2D labeling launch", e);
     return LBL_OK;
 }
 
 static lbl_status run2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &im, hipStream_t s, uint32_t *status)
 {
     bool fresh = false;
-    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, im.h, im.w, ctx->opt.mode), &fresh);
+    lbl_status st = ensure(ctx, &ctx->ws, &ctx->ws_bytes, workspace2d(alg, conn, im.h, im.w, ctx->opt.mode), &fresh);
     if (st != LBL_OK) return st;
     // every workspace starts with the count slots, which runs leave at zero
     if (fresh) LBL_HIP(ctx, hipMemsetAsync(ctx->ws, 0, kCountBytes, s));
@@ -357,12 +409,14 @@ lbl_status lbl::label2d_in_workspace(lbl_ctx *ctx, lbl_alg alg, const uint8_t *d
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     if (!d_px || !d_labels) return fail(ctx, LBL_EINVAL, "null buffer");
     if (!ws) return fail(ctx, LBL_EDEVICE, "no workspace: " + ctx->scratch_error);
+    if (alg != LBL_BLEQ && alg != LBL_LBEQ)
+        return fail(ctx, LBL_EUNSUPPORTED, "the allocation-free entry point runs bleq and lbeq only");
     int c = 0;
     lbl_status st;
     if ((st = normalize_conn2d(ctx, alg, conn, &c)) != LBL_OK) return st;
-    if ((st = check_size2d(ctx, alg, h, w, pitch_bytes)) != LBL_OK) return st;
+    if ((st = check_size2d(ctx, alg, c, h, w, pitch_bytes)) != LBL_OK) return st;
     if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
-    const size_t need = workspace2d(alg, h, w, ctx->opt.mode);
+    const size_t need = workspace2d(alg, c, h, w, ctx->opt.mode);
     if (need > ws_bytes)
         return fail(ctx, LBL_ENOMEM, "image needs " + std::to_string(need) + " bytes of workspace, more than the " +
                                          std::to_string(ws_bytes) + " available");
@@ -383,8 +437,10 @@ static lbl_status run3d(lbl_ctx *ctx, lbl_alg alg, const Image3D &im, hipStream_
     KernelTimer *kt = ctx->profiling ? &ctx->timer : nullptr;
     Options opt = ctx->opt;
     opt.status = status;
-    hipError_t e = alg == LBL_BLEQ ? bleq3d_run(im, opt, ctx->ws, s, kt) : lbeq3d_run(im, opt, ctx->ws, s, kt);
-    if (e != hipSuccess) return hip_fail(ctx, alg == LBL_BLEQ ? "bleq3d launch" : "lbeq3d launch", e);
+    hipError_t e = alg == LBL_BLEQ  ? bleq3d_run(im, opt, ctx->ws, s, kt)
+                   : alg == LBL_BIN ? bin3d_run(im, opt, ctx->ws, s, kt)
+                                    : lbeq3d_run(im, opt, ctx->ws, s, kt);
+    if (e != hipSuccess) return hip_fail(ctx, "3D labeling launch", e);
     return LBL_OK;
 }
 
@@ -398,7 +454,7 @@ extern "C" lbl_status lbl_label2d_dev(lbl_ctx *ctx, lbl_alg alg, const uint8_t *
     int c = 0;
     lbl_status st;
     if ((st = normalize_conn2d(ctx, alg, conn, &c)) != LBL_OK) return st;
-    if ((st = check_size2d(ctx, alg, h, w, pitch_bytes)) != LBL_OK) return st;
+    if ((st = check_size2d(ctx, alg, c, h, w, pitch_bytes)) != LBL_OK) return st;
     if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
     DeviceGuard g(ctx->device);
     Image2D im{d_px, pitch_bytes, h, w, d_labels, d_ncomp};
@@ -449,7 +505,7 @@ extern "C" lbl_status lbl_label2d(lbl_ctx *ctx, lbl_alg alg, const uint8_t *px, 
     int c = 0;
     lbl_status st;
     if ((st = normalize_conn2d(ctx, alg, conn, &c)) != LBL_OK) return st;
-    if ((st = check_size2d(ctx, alg, h, w, pitch_bytes)) != LBL_OK) return st;
+    if ((st = check_size2d(ctx, alg, c, h, w, pitch_bytes)) != LBL_OK) return st;
     if ((st = check_mode(ctx, alg, false)) != LBL_OK) return st;
     DeviceGuard g(ctx->device);
     const size_t n = (size_t)h * w;
@@ -506,8 +562,8 @@ extern "C" lbl_status lbl_label2d_each(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
     for (uint32_t i = 0; i < n; ++i) {
         const lbl_image2d &im = images[i];
         if (!im.px || !im.labels) return fail(ctx, LBL_EINVAL, "lbl_label2d_each: null buffer");
-        if ((st = check_size2d(ctx, alg, im.h, im.w, im.pitch_bytes)) != LBL_OK) return st;
-        need = std::max(need, workspace2d(alg, im.h, im.w, ctx->opt.mode));
+        if ((st = check_size2d(ctx, alg, c, im.h, im.w, im.pitch_bytes)) != LBL_OK) return st;
+        need = std::max(need, workspace2d(alg, c, im.h, im.w, ctx->opt.mode));
     }
     DeviceGuard g(ctx->device);
     hipStream_t s = pick_stream(ctx, stream);

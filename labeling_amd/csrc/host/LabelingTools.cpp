@@ -94,14 +94,29 @@ TTime IOCLLabeling3D::Label(const TImage &pixels, TImage &labels, char, TCoheren
 
 const std::map<std::string, Algs> &AlgList()
 {
+    // InitAlgs (main.cpp:99-131): names, descriptions and which factories exist
     static const std::map<std::string, Algs> list = [] {
         std::map<std::string, Algs> m;
-        m.emplace("lbeq", Algs{"Label equivalence by Kalentev et.al. 2012", nullptr,
+        m.emplace("bin", Algs{"Simple binarization", [] { return std::make_shared<TBinLabeling>(); },
+                              [](bool gpu) { return std::make_shared<TOCLBinLabeling>(gpu); }, nullptr,
+                              [](bool gpu) { return std::make_shared<TOCLBinLabeling3D>(gpu); }});
+        m.emplace("he-run", Algs{"Run labeling by He et.al. 2008", [] { return std::make_shared<TRunLabeling>(); },
+                                 nullptr, nullptr, nullptr});
+        m.emplace("gr-block", Algs{"Block labeling by Grana et.al. 2010",
+                                   [] { return std::make_shared<TBlockGranaLabeling>(); }, nullptr, nullptr, nullptr});
+        m.emplace("ocv", Algs{"OpenCV labeling", [] { return std::make_shared<TOpenCVLabeling>(); }, nullptr, nullptr,
+                              nullptr});
+        m.emplace("lbeq", Algs{"Label equivalence by Kalentev et.al. 2012",
+                               [] { return std::make_shared<TLabelDistribution>(); },
                                [](bool gpu) { return std::make_shared<TOCLLabelDistribution>(gpu); }, nullptr,
                                [](bool gpu) { return std::make_shared<TOCLLabelEquivalence3D>(gpu); }});
-        m.emplace("bleq", Algs{"Block equivalence by Zavalishin et.al. 2016", nullptr,
+        m.emplace("bleq", Algs{"Block equivalence by Zavalishin et.al. 2016",
+                               [] { return std::make_shared<TLabelEquivalenceX2>(); },
                                [](bool gpu) { return std::make_shared<TOCLLabelEquivalenceX2>(gpu); }, nullptr,
                                [](bool gpu) { return std::make_shared<TOCLBlockEquivalence3D>(gpu); }});
+        m.emplace("runeq", Algs{"Run equivalence by Bekhtin et.al. 2015",
+                                [] { return std::make_shared<TRunEqivLabeling>(); },
+                                [](bool gpu) { return std::make_shared<TOCLRunEquivLabeling>(gpu); }, nullptr, nullptr});
         return m;
     }();
     return list;

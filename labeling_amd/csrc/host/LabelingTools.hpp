@@ -1,6 +1,6 @@
 // LabelingTools.hpp -- C++ host interface mirroring the reference's labeling
 // framework (LabelingTools.hpp:35-177) and algorithm registry (main.cpp:88-131,
-// 373-395) for the two GPU methods provided here, implemented over the C ABI in
+// 373-395) with every method of that registry, implemented over the C ABI in
 // include/labeling_amd.h.
 //
 // Same names, argument meaning and error behaviour as the reference:
@@ -99,8 +99,11 @@ protected:
     IOCLLabeling3D(lbl_alg alg, bool runOnGPU) : IOCLLabeling(alg, runOnGPU) {}
 };
 
-// The four GPU classes of the reference's registry entries "bleq" and "lbeq"
-// (LabelingAlgs.hpp:215-264, 339-390).
+// The GPU classes of the reference's registry (LabelingAlgs.hpp:196-390).
+class TOCLBinLabeling final : public IOCLLabeling {
+public:
+    explicit TOCLBinLabeling(bool runOnGPU = true) : IOCLLabeling(LBL_BIN, runOnGPU) {}
+};
 class TOCLLabelDistribution final : public IOCLLabeling {
 public:
     explicit TOCLLabelDistribution(bool runOnGPU = true) : IOCLLabeling(LBL_LBEQ, runOnGPU) {}
@@ -108,6 +111,14 @@ public:
 class TOCLLabelEquivalenceX2 final : public IOCLLabeling {
 public:
     explicit TOCLLabelEquivalenceX2(bool runOnGPU = true) : IOCLLabeling(LBL_BLEQ, runOnGPU) {}
+};
+class TOCLRunEquivLabeling final : public IOCLLabeling {
+public:
+    explicit TOCLRunEquivLabeling(bool runOnGPU = true) : IOCLLabeling(LBL_RUNEQ, runOnGPU) {}
+};
+class TOCLBinLabeling3D final : public IOCLLabeling3D {
+public:
+    explicit TOCLBinLabeling3D(bool runOnGPU = true) : IOCLLabeling3D(LBL_BIN, runOnGPU) {}
 };
 class TOCLLabelEquivalence3D final : public IOCLLabeling3D {
 public:
@@ -118,8 +129,41 @@ public:
     explicit TOCLBlockEquivalence3D(bool runOnGPU = true) : IOCLLabeling3D(LBL_BLEQ, runOnGPU) {}
 };
 
-// Registry (main.cpp:88-131).  cpu / cpu3d are empty: this library provides the
-// GPU paths only; the reference's CPU classes are restated by the test oracle.
+// The reference's CPU classes (LabelingAlgs.hpp:29-190), computed on the GPU
+// with the label values of the CPU methods (include/labeling_amd.h describes
+// each method's values); `threads` is ignored (it sets OpenMP threads in the
+// reference).  They are what SetLabelingAlg returns without -g / -u.
+class TBinLabeling final : public IOCLLabeling {
+public:
+    TBinLabeling() : IOCLLabeling(LBL_BIN, true) {}
+};
+class TOpenCVLabeling final : public IOCLLabeling {
+public:
+    TOpenCVLabeling() : IOCLLabeling(LBL_OCV, true) {}
+};
+class TBlockGranaLabeling final : public IOCLLabeling {
+public:
+    TBlockGranaLabeling() : IOCLLabeling(LBL_GRBLOCK, true) {}
+};
+class TRunLabeling final : public IOCLLabeling {
+public:
+    TRunLabeling() : IOCLLabeling(LBL_HERUN, true) {}
+};
+class TLabelDistribution final : public IOCLLabeling {
+public:
+    TLabelDistribution() : IOCLLabeling(LBL_LBEQ, true) {}
+};
+class TLabelEquivalenceX2 final : public IOCLLabeling {
+public:
+    TLabelEquivalenceX2() : IOCLLabeling(LBL_BLEQ, true) {}
+};
+class TRunEqivLabeling final : public IOCLLabeling {
+public:
+    TRunEqivLabeling() : IOCLLabeling(LBL_RUNEQ, true) {}
+};
+
+// Registry (main.cpp:88-131): every entry of ALG_LIST with the reference's
+// factories; a factory the reference leaves empty is empty here too.
 struct Algs {
     std::string descr;
     std::function<std::shared_ptr<ILabeling>()> cpu;

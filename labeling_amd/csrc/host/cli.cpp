@@ -141,7 +141,13 @@ void PrintHelp()
                  "  -o <out_path>   : Output path (existing directory)\n"
                  "  -a <algorithm>  : Labeling algorithm:\n";
     for (const auto &a : AlgList())
-        std::cout << "            " << a.first << "     : " << a.second.descr << " (OCL+3D)\n";
+    {
+        std::string caps;
+        if (a.second.cpu) caps += " CPU";
+        if (a.second.ocl) caps += " GPU";
+        if (a.second.ocl3d) caps += " GPU-3D";
+        std::cout << "            " << a.first << "     : " << a.second.descr << " [" << caps.substr(1) << "]\n";
+    }
     std::cout << "  -3           : Treat input sequence as a single 3D image\n"
                  "  -g           : Run algorithm on GPU\n"
                  "  -u           : Run algorithm in OpenCL mode on CPU (not available)\n"

@@ -83,6 +83,18 @@ hipError_t bleq2d_run(const Image2D &im, const Options &opt, void *ws, hipStream
 hipError_t bleq2d_run_conn4(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 size_t bleq2d_conn4_workspace(uint32_t h, uint32_t w, bool canonical);
 hipError_t bleq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+
+// The reference registry's other methods (ALG_LIST, main.cpp:103-130), on the
+// same block forests: gr-block (block-order numbering), he-run / runeq (run
+// numbering; he: He et al., else run equivalence), and bin (foreground -> 1,
+// 2D and 3D; ws holds the count slots).
+size_t grblock2d_workspace(uint32_t h, uint32_t w);
+hipError_t grblock2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
+size_t runs2d_workspace(uint32_t h, uint32_t w, bool conn8, bool canonical);
+hipError_t runs2d_run(const Image2D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt, bool he,
+                      int conn);
+hipError_t bin2d_run(const Image2D &im, void *ws, hipStream_t s, KernelTimer *kt);
+hipError_t bin3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 hipError_t lbeq2d_run(const Image2D &im, int conn, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 hipError_t lbeq3d_run(const Image3D &im, const Options &opt, void *ws, hipStream_t s, KernelTimer *kt);
 

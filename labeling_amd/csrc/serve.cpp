@@ -274,7 +274,8 @@ extern "C" lbl_status lbl_server_submit(lbl_server *srv, const lbl_request *req,
     if (!srv || !req || !ticket) return LBL_EINVAL;
     const lbl_request &q = *req;
     if (!q.px || !q.labels || !q.h || !q.w) return LBL_EINVAL;
-    if (q.alg != LBL_BLEQ && q.alg != LBL_LBEQ) return LBL_EINVAL;
+    if (q.alg != LBL_BLEQ && q.alg != LBL_LBEQ)   // the server runs the two union-find methods
+        return (int)q.alg > (int)LBL_LBEQ && (int)q.alg <= (int)LBL_OCV ? LBL_EUNSUPPORTED : LBL_EINVAL;
     if (q.d == 0) {
         if (q.alg == LBL_BLEQ && q.conn != 0 && q.conn != 8) return q.conn == 4 ? LBL_EUNSUPPORTED : LBL_EINVAL;
         if (q.alg == LBL_LBEQ && q.conn != 0 && q.conn != 4 && q.conn != 8) return LBL_EINVAL;

@@ -115,6 +115,33 @@ int64_t orc_exhaustive2d(int64_t h, int64_t w, int alg, int conn, uint64_t *firs
  * orc_lbeq3d (alg 1) versus orc_bfs3d with 26-connectivity. */
 int64_t orc_exhaustive3d(int64_t d, int64_t h, int64_t w, int alg, uint64_t *first_fail);
 
+/* ---- the registry's other methods (oracle_registry.c; main.cpp:103-130) ---- */
+
+/* Run equivalence (TRunEqivLabeling, LabelingAlgs.cpp:535-810; its OpenCL
+ * kernels REFindRunsKernel .. RELabelKernel, LabelingAlgs.cl:348-616, compute
+ * the same on the padded image).  Runs of each row are numbered in slots of
+ * `slots` per row (the reference: w >> 1; a row with more runs returns -2
+ * instead of spilling into the next row's slots); runs of adjacent rows join
+ * where IsNeib (:605-613) finds a shared column; the Scan / Analyze loop runs
+ * sequentially to its fixed point.  labels: 1 + row * slots + the index of the
+ * component's first run in its row. */
+int orc_runeq2d(const uint8_t *px, int64_t h, int64_t w, int64_t slots, uint32_t *labels, uint32_t *iterations);
+
+/* He et al.'s run labeling (TRunLabeling::DoLabel / SetRunLabel,
+ * LabelingAlgs.cpp:58-205): conpix 1 for 8-connectivity (coh == COH_8), 0 for
+ * 4.  quirks = 1: as written: the last row is not scanned (Row < Bottom =
+ * rows - 1, :150-153), a run reaching the last column loses its last pixel and
+ * a run starting there is dropped (:158-171).  quirks = 0: every pixel. */
+int orc_herun2d(const uint8_t *px, int64_t h, int64_t w, int conpix, int quirks, uint32_t *labels);
+
+/* Block-based two-pass 8-connected labeling after Grana et al. (the decisions
+ * of cvLabelingImageLab, cvlabeling_imagelab.cpp:57-1567, without its decision
+ * tree): 2x2 blocks in raster order take the provisional label of the earlier
+ * neighbour blocks they touch and merge the others (the smallest label stays
+ * the representative), else a new label; labels are renumbered 1..N in order
+ * of their representative (:1523-1531).  Returns N. */
+int64_t orc_grblock2d(const uint8_t *px, int64_t h, int64_t w, uint32_t *labels);
+
 #ifdef __cplusplus
 }
 #endif

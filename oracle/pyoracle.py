@@ -38,6 +38,9 @@ def load():
             "orc_bleq3d_conn": (i64, [P, i64, i64, i64, P]),
             "orc_exhaustive2d": (i64, [i64, i64, i32, i32, ctypes.POINTER(ctypes.c_uint64)]),
             "orc_exhaustive3d": (i64, [i64, i64, i64, i32, ctypes.POINTER(ctypes.c_uint64)]),
+            "orc_runeq2d": (i32, [P, i64, i64, i64, P, pu32]),
+            "orc_herun2d": (i32, [P, i64, i64, i32, i32, P]),
+            "orc_grblock2d": (i64, [P, i64, i64, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -168,3 +171,57 @@ def exhaustive3d(d, h, w, alg):
     first = ctypes.c_uint64(0)
     n = load().orc_exhaustive3d(d, h, w, alg, ctypes.byref(first))
     return int(n), int(first.value)
+
+
+# ---- the registry's other methods (oracle_registry.c) --------------------------------
+
+def runeq2d(img, reference=False, slots=None):
+    """Restated run equivalence (TRunEqivLabeling, LabelingAlgs.cpp:535-810).
+    reference=False: the unpadded image with ceil(w/2) run slots per row (the
+    CPU values, w >> 1 slots, for even w); reference=True: the reference GPU's
+    padded image (orc_pad2 in both dimensions) with P2(w)/2 slots, cropped."""
+    img = _u8(img)
+    h, w = img.shape
+    L = load()
+    if reference:
+        ph, pw = L.orc_pad2(h), L.orc_pad2(w)
+        src = np.zeros((ph, pw), np.uint8)
+        src[:h, :w] = img
+        ns = pw // 2
+    else:
+        src, ph, pw = img, h, w
+        ns = slots if slots is not None else (w + 1) // 2
+    out = np.empty((ph, pw), dtype=np.uint32)
+    it = ctypes.c_uint32()
+    rc = L.orc_runeq2d(src.ctypes.data, ph, pw, ns, out.ctypes.data, ctypes.byref(it))
+    if rc != 0:
+        raise ValueError(f"orc_runeq2d failed ({rc})")
+    return np.ascontiguousarray(out[:h, :w])
+
+
+def herun2d(img, conn=4, quirks=False):
+    """Restated He et al. run labeling (TRunLabeling, LabelingAlgs.cpp:58-205);
+    conn 8 extends run contacts by one pixel (ConPix, :140)."""
+    img = _u8(img)
+    h, w = img.shape
+    out = np.empty((h, w), dtype=np.uint32)
+    rc = load().orc_herun2d(img.ctypes.data, h, w, 1 if conn == 8 else 0, int(quirks), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("orc_herun2d failed")
+    return out
+
+
+def grblock2d(img):
+    """Block-based two-pass 8-connected labeling after Grana et al.; (labels, N)."""
+    img = _u8(img)
+    h, w = img.shape
+    out = np.empty((h, w), dtype=np.uint32)
+    n = load().orc_grblock2d(img.ctypes.data, h, w, out.ctypes.data)
+    if n < 0:
+        raise ValueError("orc_grblock2d failed")
+    return out, int(n)
+
+
+def bin2d(img):
+    """TBinLabeling (LabelingAlgs.cpp:23-36): 1 on foreground."""
+    return (np.asarray(img) != 0).astype(np.uint32)

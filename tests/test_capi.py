@@ -23,10 +23,12 @@ def test_library_exports_every_header_symbol():
 def test_registry_names_mirror_alg_list():
     lib = L.load()
     out = ctypes.c_int(-1)
-    assert lib.lbl_alg_from_name(b"bleq", ctypes.byref(out)) == L.OK and out.value == L.BLEQ
-    assert lib.lbl_alg_from_name(b"lbeq", ctypes.byref(out)) == L.OK and out.value == L.LBEQ
-    for other in (b"bin", b"runeq", b"he-run", b"gr-block", b"ocv"):
-        assert lib.lbl_alg_from_name(other, ctypes.byref(out)) == L.EUNSUPPORTED
+    # every ALG_LIST name of the reference (main.cpp:103-130)
+    for name, alg in (("bleq", L.BLEQ), ("lbeq", L.LBEQ), ("bin", L.BIN), ("runeq", L.RUNEQ), ("he-run", L.HERUN),
+                      ("gr-block", L.GRBLOCK), ("ocv", L.OCV)):
+        out.value = -1
+        assert lib.lbl_alg_from_name(name.encode(), ctypes.byref(out)) == L.OK and out.value == alg
+        assert L.ALGS[name] == alg
     assert lib.lbl_alg_from_name(b"nope", ctypes.byref(out)) == L.EINVAL
 
 

@@ -26,21 +26,37 @@ def checker(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("alg", ["bleq", "lbeq"])
-def test_registry_2d(checker, tmp_path, alg):
+# (name, device, expected labels of a binary image): every ALG_LIST entry through
+# the factory main.cpp would pick (-g for the GPU classes, none for the CPU ones)
+REGISTRY = [
+    ("bleq", "gpu", lambda b: O.canonical(O.bfs2d(b, 8)[0])[0], True),
+    ("lbeq", "gpu", lambda b: O.canonical(O.bfs2d(b, 4)[0])[0], True),
+    ("runeq", "gpu", lambda b: O.runeq2d(b), False),
+    ("bin", "gpu", lambda b: O.bin2d(b), False),
+    ("bleq", "cpu", lambda b: O.canonical(O.bfs2d(b, 8)[0])[0], True),
+    ("he-run", "cpu", lambda b: O.herun2d(b, 4), False),
+    ("gr-block", "cpu", lambda b: O.grblock2d(b)[0], False),
+    ("ocv", "cpu", lambda b: O.canonical(O.bfs2d(b, 4)[0])[0], False),
+    ("runeq", "cpu", lambda b: O.runeq2d(b), False),
+]
+
+
+@pytest.mark.parametrize("entry", REGISTRY, ids=lambda e: f"{e[0]}-{e[1]}")
+def test_registry_2d(checker, tmp_path, entry):
+    alg, device, expect, canonical = entry
     img = L.bernoulli((301, 517), 0.5, seed=3)
     img[img > 0] = 9      # binarised by RGB2Gray's v > 8 rule
     img[0, :5] = 8        # 8 is background after binarisation
     raw = tmp_path / "in.raw"
     img.tofile(raw)
     out = tmp_path / "out.raw"
-    r = subprocess.run([checker, str(raw), "301", "517", str(out), "0", alg], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([checker, str(raw), "301", "517", str(out), "0", alg, device], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     lab = np.fromfile(out, dtype=np.uint32).reshape(img.shape)
     binary = np.where(img > 8, 255, 0).astype(np.uint8)
-    conn = 8 if alg == "bleq" else 4
-    np.testing.assert_array_equal(O.canonical(lab)[0], O.bfs2d(binary, conn)[0])
+    got = O.canonical(lab)[0] if canonical else lab
+    np.testing.assert_array_equal(got, expect(binary))
 
 
 def test_registry_3d(checker, tmp_path):
@@ -86,6 +102,10 @@ def test_cli_file_dir_and_3d(tmp_path):
     r = subprocess.run([exe, "-a", "bleq", "-g", "-c", "4", "-i", str(tmp_path / "a.pgm")], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode != 0 and "4-connectivity" in r.stdout
-    r = subprocess.run([exe, "-a", "runeq", "-g", "-i", str(tmp_path / "a.pgm")], capture_output=True, text=True,
+    # he-run has no GPU (OpenCL) factory in the reference: -g fails, the CPU entry runs
+    r = subprocess.run([exe, "-a", "he-run", "-g", "-i", str(tmp_path / "a.pgm")], capture_output=True, text=True,
                        timeout=120)
-    assert r.returncode != 0
+    assert r.returncode != 0 and "doesn't support" in r.stdout
+    r = subprocess.run([exe, "-a", "he-run", "-i", str(tmp_path / "a.pgm")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "Processing time" in r.stdout, r.stdout
