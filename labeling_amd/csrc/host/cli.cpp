@@ -1,13 +1,18 @@
 // cli.cpp -- `labeling` command line, mirroring the reference's main.cpp flags
-// (-i -o -a -3 -g -u -j -l -c -h; main.cpp:344-369,399-447) for the GPU
-// methods "bleq" and "lbeq".
+// (-i -o -a -3 -g -u -j -l -c -h; main.cpp:344-369,399-447) for every method
+// of its registry.
 //
-// Image files are binary PGM (P5, 8-bit) instead of OpenCV-decoded formats; a
-// 3D image is a directory of equally sized PGM slices in sorted file-name order
-// (Read3DImage, main.cpp:172-212), stored as {rows, cols, slices} with the slice
-// index fastest like the reference.  Binarisation is the reference's v > 8
-// (LabelingTools.cpp:46).  Output (-o, an existing directory): colourised PPM
-// images (LabelsToRGB / Write3DLabels, main.cpp:250-305).  Timing lines follow
+// Image files are read and written as the reference's driver does through
+// OpenCV (imageio.hpp): a 2D input is read in color (cv::imread,
+// main.cpp:322,477) and turned gray by ILabeling::RGB2Gray's cv::cvtColor
+// (COLOR_RGB2GRAY applied to OpenCV's BGR order, LabelingTools.cpp:43-44);
+// a 3D input is a directory of equally sized slices read as gray
+// (IMREAD_GRAYSCALE, main.cpp:172-212) in sorted file-name order, stored as
+// {rows, cols, slices} with the slice index fastest like the reference.
+// Binarisation is the reference's v > 8 (LabelingTools.cpp:46).  Output (-o, an
+// existing directory): colourised labels under the input's file name, in its
+// format (main.cpp:329,486), and for 3D one PNG per slice, 1.png, 2.png, ...
+// (Write3DLabels, main.cpp:290-305).  Timing lines follow
 // main.cpp:334-339,451-462 (per-image ms, min/avg/max over -l cycles).
 #include <algorithm>
 #include <cstdio>
@@ -19,6 +24,7 @@
 #include <sys/stat.h>
 
 #include "LabelingTools.hpp"
+#include "imageio.hpp"
 
 using namespace LabelingTools;
 
@@ -52,26 +58,18 @@ std::string base_name(const std::string &p)
     return k == std::string::npos ? p : p.substr(k + 1);
 }
 
-// Binary PGM (P5, maxval <= 255).
-TImage read_pgm(const std::string &path)
+// A 2D image as the reference's driver hands it to Label(): cv::imread's BGR
+// pixels through cvtColor(COLOR_RGB2GRAY), whose weights 0.299 / 0.587 / 0.114
+// (14-bit fixed point) then fall on B, G, R (LabelingTools.cpp:43-44).
+TImage read_2d(const std::string &path)
 {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) throw std::runtime_error("cannot open " + path);
-    std::string magic;
-    f >> magic;
-    auto next_int = [&]() {
-        int v;
-        while (f >> std::ws && f.peek() == '#') f.ignore(1 << 20, '\n');
-        f >> v;
-        return v;
-    };
-    if (magic != "P5") throw std::runtime_error(path + ": not a binary PGM (P5) file");
-    int w = next_int(), h = next_int(), maxv = next_int();
-    if (w <= 0 || h <= 0 || maxv <= 0 || maxv > 255) throw std::runtime_error(path + ": unsupported PGM header");
-    f.get();
-    TImage img(h, w, TImage::U8);
-    f.read(reinterpret_cast<char *>(img.data()), (std::streamsize)img.total());
-    if (!f) throw std::runtime_error(path + ": truncated PGM data");
+    const io::Image8 c = io::imread_color(path);
+    TImage img(c.h, c.w, TImage::U8);
+    uint8_t *d = img.data();
+    for (size_t i = 0; i < (size_t)c.w * c.h; ++i) {
+        const uint8_t *s = &c.px[i * 3];
+        d[i] = (uint8_t)((s[0] * 4899 + s[1] * 9617 + s[2] * 1868 + 8192) >> 14);
+    }
     return img;
 }
 
@@ -81,7 +79,7 @@ std::vector<std::string> find_files(const std::string &dir)
     if (DIR *d = opendir(dir.c_str())) {
         while (dirent *e = readdir(d)) {
             std::string n = e->d_name;
-            if (n.size() > 4 && n.substr(n.size() - 4) == ".pgm" && is_file(dir + "/" + n)) out.push_back(dir + "/" + n);
+            if (io::is_image_name(n) && is_file(dir + "/" + n)) out.push_back(dir + "/" + n);   // main.cpp:134-152
         }
         closedir(d);
     }
@@ -95,13 +93,13 @@ TImage read_3d(const std::string &dir)
     if (files.empty()) throw std::runtime_error("Wrong input path (assumed directory with 3D image slices)");
     TImage vol;
     for (size_t k = 0; k < files.size(); ++k) {
-        TImage s = read_pgm(files[k]);
-        if (vol.empty()) vol = TImage(s.rows(), s.cols(), (int)files.size(), TImage::U8);
-        if (s.rows() != vol.size(0) || s.cols() != vol.size(1))
+        const io::Image8 s = io::imread_gray(files[k]);
+        if (vol.empty()) vol = TImage(s.h, s.w, (int)files.size(), TImage::U8);
+        if (s.h != vol.size(0) || s.w != vol.size(1))
             throw std::runtime_error("Cannot read 3D image: slice sizes do not match");
-        for (int i = 0; i < s.rows(); ++i)
-            for (int j = 0; j < s.cols(); ++j)
-                vol.data()[((size_t)i * vol.size(1) + j) * vol.size(2) + k] = s.data()[(size_t)i * s.cols() + j];
+        for (int i = 0; i < s.h; ++i)
+            for (int j = 0; j < s.w; ++j)
+                vol.data()[((size_t)i * vol.size(1) + j) * vol.size(2) + k] = s.px[(size_t)i * s.w + j];
     }
     return vol;
 }
@@ -112,17 +110,21 @@ void color_of(uint32_t l, uint8_t rgb[3])
     rgb[0] = l ? (uint8_t)(h >> 24) : 0;
     rgb[1] = l ? (uint8_t)(h >> 16) : 0;
     rgb[2] = l ? (uint8_t)(h >> 8) : 0;
+    if (l && !(rgb[0] | rgb[1] | rgb[2])) rgb[0] = 1;   // black is the background's
 }
 
-void write_ppm(const std::string &path, const uint32_t *lab, int rows, int cols, size_t stride, size_t step)
+// LabelsToRGB (main.cpp:250-286): a colour per label, black background; the
+// reference draws colours with std::rand, here a hash of the label gives them.
+io::Image8 labels_to_bgr(const uint32_t *lab, int rows, int cols, size_t stride, size_t step)
 {
-    std::ofstream f(path, std::ios::binary);
-    f << "P6\n" << cols << " " << rows << "\n255\n";
-    std::vector<uint8_t> row((size_t)cols * 3);
-    for (int i = 0; i < rows; ++i) {
-        for (int j = 0; j < cols; ++j) color_of(lab[(size_t)i * stride + (size_t)j * step], &row[(size_t)j * 3]);
-        f.write(reinterpret_cast<const char *>(row.data()), (std::streamsize)row.size());
-    }
+    io::Image8 o;
+    o.w = cols;
+    o.h = rows;
+    o.c = 3;
+    o.px.resize((size_t)rows * cols * 3);
+    for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j) color_of(lab[(size_t)i * stride + (size_t)j * step], &o.px[((size_t)i * cols + j) * 3]);
+    return o;
 }
 
 struct ImgTime {   // main.cpp:46-84
@@ -137,7 +139,7 @@ struct ImgTime {   // main.cpp:46-84
 void PrintHelp()
 {
     std::cout << "Usage: labeling [options]\n\nOptions:\n"
-                 "  -i <input_path> : Input file (binary PGM) or path\n"
+                 "  -i <input_path> : Input file or path (.jpg .jpeg .bmp .png .tif .tiff, PNM)\n"
                  "  -o <out_path>   : Output path (existing directory)\n"
                  "  -a <algorithm>  : Labeling algorithm:\n";
     for (const auto &a : AlgList())
@@ -203,8 +205,9 @@ int Run(const Options &o)
         PrintTime(o.inPath, t, o);
         if (want_out)
             for (int k = 0; k < labels.size(2); ++k)
-                write_ppm(o.outPath + "/" + std::to_string(k + 1) + ".ppm", labels.ptr<uint32_t>() + k, labels.size(0),
-                          labels.size(1), (size_t)labels.size(1) * labels.size(2), (size_t)labels.size(2));
+                io::imwrite(o.outPath + "/" + std::to_string(k + 1) + ".png",
+                            labels_to_bgr(labels.ptr<uint32_t>() + k, labels.size(0), labels.size(1),
+                                          (size_t)labels.size(1) * labels.size(2), (size_t)labels.size(2)));
         return 0;
     }
     std::vector<std::string> files;
@@ -214,7 +217,7 @@ int Run(const Options &o)
     ImgTime all;
     size_t count = 0;
     for (const auto &f : files) {
-        TImage img = ILabeling::RGB2Gray(read_pgm(f)), labels;
+        TImage img = ILabeling::RGB2Gray(read_2d(f)), labels;
         ImgTime t;
         for (int c = 0; c < o.cycles; ++c) {
             TTime us = alg->Label(img, labels, (char)o.numThreads, o.coh);
@@ -225,8 +228,8 @@ int Run(const Options &o)
         else std::cout << "Processing image " << ++count << "/" << files.size() << " (" << base_name(f) << ") "
                        << t.Avg() / 1000 << " ms\n";
         if (want_out)
-            write_ppm(o.outPath + "/" + base_name(f) + ".ppm", labels.ptr<uint32_t>(), labels.rows(), labels.cols(),
-                      (size_t)labels.cols(), 1);
+            io::imwrite(o.outPath + "/" + base_name(f),
+                        labels_to_bgr(labels.ptr<uint32_t>(), labels.rows(), labels.cols(), (size_t)labels.cols(), 1));
     }
     if (files.size() > 1)
         std::cout << "\nMin processing time: " << all.Min() / 1000.0 << " ms\nAvg processing time: "
