@@ -17,8 +17,9 @@ of the kernels, measured with HIP events that the library records between its
 kernels on the launch stream; peak 8 TB/s.  `cpu_baseline` is the oracle's
 OpenMP restatement of the reference CPU labeler (TLabelEquivalenceX2) on this
 host.  `extra` holds the other configurations: 3D 512^3, label equivalence,
-the batched 4096-image workload (one hipGraph launch vs per-image launches)
-and the continuous-batching server (p50/p99 latency at a fixed request rate).
+the reference registry's other methods on the same images, the batched
+4096-image workload (one hipGraph launch vs per-image launches) and the
+continuous-batching server (p50/p99 latency at a fixed request rate).
 """
 import argparse
 import json
@@ -166,8 +167,28 @@ def bench_2d(L, torch, lab, a):
             lbeq[f"conn{conn}_gpx_s"] = round(px * 12 / (ms2 / 1e3) / 1e9, 2)
         out["lbeq2d"] = lbeq
         out["canonical"] = canonical_line(L, torch, lab, stream, step, names, px, a)
+        if "registry" not in set(filter(None, a.skip.split(","))):
+            out["registry"] = registry_line(torch, lab, stream, step, names, px)
     del d_in
     return out
+
+
+def registry_line(torch, lab, stream, step, names, px, steps=12):
+    """The reference registry's other methods (ALG_LIST, main.cpp:103-130) on
+    the same six images, each in its default label values: Gpixel/s over
+    `steps` timed steps (every input twice) and the per-kernel times."""
+    res = {}
+    for alg, conn in (("bin", 0), ("runeq", 0), ("he-run", 4), ("he-run", 8), ("gr-block", 0), ("ocv", 4),
+                      ("ocv", 8)):
+        def run(i, alg=alg, conn=conn):
+            step(i, alg, conn)
+
+        ms, _ = timed_steps(torch, stream, run, len(names), 2, steps)
+        kavg, _ = kernel_breakdown(lab, run, names, len(names))
+        res[f"{alg}_conn{conn}" if conn else alg] = {
+            "gpx_s": round(px * steps / (ms / 1e3) / 1e9, 2), "ms_per_step": round(ms / steps, 4),
+            "kernels_us": {k: round(v, 2) for k, v in kavg.items()}}
+    return res
 
 
 def canonical_line(L, torch, lab, stream, step, names, units, a, steps=30):
@@ -419,13 +440,26 @@ def cpu_baseline(L, inputs, seconds_budget):
     for alg, fn in (("bleq", O.bleq3d), ("lbeq", O.lbeq3d)):
         _, info = fn(vol, threads=threads, with_info=True)
         vol3[f"{alg}3d_512_p0.3_gvox_s"] = round(vol.size / info["seconds"] / 1e9, 4)
+    # the other registry methods' restatements (single-threaded, whole call) on
+    # p=0.5 images: 1024^2, and 512^2 for He et al.'s run labeling, whose label
+    # set merges (relabelling a whole set, as the reference does) grow
+    # quadratically with the image
+    registry = {}
+    for name, side, fn in (("runeq", 1024, lambda im: O.runeq2d(im)), ("gr-block", 1024, lambda im: O.grblock2d(im)),
+                           ("he-run_conn4", 512, lambda im: O.herun2d(im, 4)),
+                           ("he-run_conn8", 512, lambda im: O.herun2d(im, 8))):
+        img = L.bernoulli((side, side), 0.5, seed=0)
+        t0 = time.perf_counter()
+        fn(img)
+        registry[f"{name}_{side}_p0.5_gpx_s"] = round(img.size / (time.perf_counter() - t0) / 1e9, 5)
     bleq = _spread(runs)
     return {"value": bleq["median"], "unit": "Gpixel/s", "cores": threads, "kind": "port",
             "sample": f"{len(runs)} labelings of the six 8192x8192 inputs (cycled) by the oracle's OpenMP "
                       f"restatement of TLabelEquivalenceX2 on the padded image, DoLabel region only; value = median",
             "cpu_model": cpu_model(), "host_cpus": host_cpus, "threads": threads,
             "bleq2d_8192_gpx_s": bleq, "bleq2d_8192_1thread_gpx_s": _spread(one),
-            "lbeq": lbeq, "restatement_3d": vol3, "seconds": round(time.perf_counter() - t_start, 1)}
+            "lbeq": lbeq, "restatement_3d": vol3, "registry_1thread": registry,
+            "seconds": round(time.perf_counter() - t_start, 1)}
 
 
 def main():
@@ -443,7 +477,7 @@ def main():
                     help="2D: run the library's autotuner on the p=0.5 input instead of loading the shipped table")
     ap.add_argument("--variant", default="", help="TILE,FLATTEN: run this bleq 2D kernel variant, no autotune")
     ap.add_argument("--quick", action="store_true", help="headline 2D line only")
-    ap.add_argument("--skip", default="", help="comma list of extras to skip: 3d,batch,serve")
+    ap.add_argument("--skip", default="", help="comma list of extras to skip: 3d,batch,serve,registry")
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -490,6 +524,8 @@ def main():
             extra["lbeq2d_8192"] = r2.pop("lbeq2d")
         if "canonical" in r2:
             extra["canonical_2d_8192"] = r2.pop("canonical")
+        if "registry" in r2:
+            extra["registry_2d_8192"] = r2.pop("registry")
         if "3d" not in skip:
             extra["volume_512"] = bench_3d(L, torch, lab, a)
         if "batch" not in skip:
