@@ -60,7 +60,11 @@ class DeviceBuffer:
         return out
 
     def zero(self):
+        # hipMemset on device memory may return before it has run (it is
+        # queued on the null stream), and the library's own streams do not
+        # wait for the null stream: finish it before anything else runs
         check(lib().hipMemset(self.addr, 0, self.nbytes), "hipMemset")
+        check(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
     def free(self):
         if self.ptr:
