@@ -343,9 +343,10 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
     const uint32_t o = a | b;
     const uint32_t st = o & ~(o << 1);               // first column of each segment-local run
     const uint32_t nb = r * TBW + j * LANE_BLOCKS;   // local index of the segment's first block
+    const uint32_t nbn = nb * kNodeBytes;            // its node name
 
     // node standing for the segment's first run (the band run's head if it continues from the left)
-    const uint32_t first = run_head(o, j, r * TBW);
+    const uint32_t first = run_head(o, j, r * TBW * kNodeBytes);
 
     so[r + 1][j + 1] = o;
     sb[r + 1][j + 1] = CONN4 ? a : b;   // the row the band below touches
@@ -358,17 +359,21 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
         if (j == LPB - 1) so[0][LPB + 1] = sb[0][LPB + 1] = 0;
     }
     {
+        constexpr uint32_t K = kNodeBytes;
         uint4 *p4 = reinterpret_cast<uint4 *>(&par[nb]);
-        p4[0] = make_uint4(first, nb + 1, nb + 2, nb + 3);
+        p4[0] = make_uint4(first, nbn + K, nbn + 2 * K, nbn + 3 * K);
 #pragma unroll
-        for (uint32_t q = 1; q < LANE_BLOCKS / 4; ++q) p4[q] = make_uint4(nb + 4 * q, nb + 4 * q + 1, nb + 4 * q + 2, nb + 4 * q + 3);
+        for (uint32_t q = 1; q < LANE_BLOCKS / 4; ++q)
+            p4[q] = make_uint4(nbn + 4 * q * K, nbn + (4 * q + 1) * K, nbn + (4 * q + 2) * K, nbn + (4 * q + 3) * K);
     }
     __syncthreads();
 
     // unions along the vertical contacts with the band above
     {
         const uint32_t ou = so[r][j + 1];
-        const SegNodes up{ou & ~(ou << 1), nb - TBW, sh[r][j + 1], last_run_node(so[r][j], nb - TBW - LANE_BLOCKS, sh[r][j]),
+        const uint32_t upn = nbn - TBW * kNodeBytes;   // name of the upper segment's first block
+        const SegNodes up{ou & ~(ou << 1), upn, sh[r][j + 1],
+                          last_run_node(so[r][j], upn - LANE_BLOCKS * kNodeBytes, sh[r][j]),
                           sh[r][j + 2]};   // only used when a contact exists (r > 0)
         SegContacts cs(a, sb[r][j + 1], sb[r][j], sb[r][j + 2], st, up.st);
         if (CONN4) cs.el = cs.er = 0;   // straight contacts only
@@ -378,17 +383,17 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
             if (!busy) {
                 if (!cs.any()) break;
                 uint32_t x;
-                const int p = cs.pop(x);
+                const int p = cs.pop_any(x);
                 const uint32_t s = hibit(st & upto(x));
-                oa = ua = s ? nb + (s >> 1) : first;
+                oa = ua = s ? node_at(nbn, s) : first;
                 ob = ub = up.node(p);
                 busy = true;
             }
             uint32_t top;
             if (!lds_rem_step(par, ua, ub, top)) continue;
             // both original nodes now point at the node where the sets met (or lower)
-            lds_min(&par[oa], top);
-            lds_min(&par[ob], top);
+            lds_min(node_entry(par, oa), top);
+            lds_min(node_entry(par, ob), top);
             busy = false;
         }
     }
@@ -411,9 +416,9 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
     uint32_t pend = 0;
 #pragma unroll
     for (uint32_t kk = 0; kk < LANE_BLOCKS; ++kk)
-        if ((st & (3u << (2 * kk))) && v[kk] != nb + kk) pend |= 1u << kk;
+        if ((st & (3u << (2 * kk))) && v[kk] != nbn + kk * kNodeBytes) pend |= 1u << kk;
     if (pend) {
-        resolve_nodes(par, nb, pend, [&](uint32_t l) { return (by0 + (l >> LOG_TBW)) * bw + bx0 + (l & (TBW - 1)); });
+        resolve_nodes(par, nbn, pend, [&](uint32_t l) { return (by0 + (l >> LOG_TBW)) * bw + bx0 + (l & (TBW - 1)); });
         load_nodes();
     }
 

@@ -157,8 +157,8 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     }
     const uint32_t o = a | b | c | e;
     const uint32_t st = o & ~(o << 1);                 // first column of each segment-local run
-    const uint32_t bar0 = bar * TBW, nb = bar0 + j * kLaneBlocks;
-    const uint32_t first = run_head(o, j, bar0);       // node standing for the segment's first run
+    const uint32_t bar0 = bar * TBW, nb = bar0 + j * kLaneBlocks, nbn = nb * kNodeBytes;   // nbn: node name
+    const uint32_t first = run_head(o, j, bar0 * kNodeBytes);   // node standing for the segment's first run
 
     // guard entries (outside the tile) read as empty segments
     for (uint32_t i = t; i < T::ENTRIES; i += T::THREADS) {
@@ -172,10 +172,12 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     se[me] = e;
     sh[me] = first;
     {
+        constexpr uint32_t K = kNodeBytes;
         uint4 *p4 = reinterpret_cast<uint4 *>(&par[nb]);
-        p4[0] = make_uint4(first, nb + 1, nb + 2, nb + 3);
+        p4[0] = make_uint4(first, nbn + K, nbn + 2 * K, nbn + 3 * K);
 #pragma unroll
-        for (uint32_t q = 1; q < kLaneBlocks / 4; ++q) p4[q] = make_uint4(nb + 4 * q, nb + 4 * q + 1, nb + 4 * q + 2, nb + 4 * q + 3);
+        for (uint32_t q = 1; q < kLaneBlocks / 4; ++q)
+            p4[q] = make_uint4(nbn + 4 * q * K, nbn + (4 * q + 1) * K, nbn + (4 * q + 2) * K, nbn + (4 * q + 3) * K);
     }
     __syncthreads();
 
@@ -199,9 +201,9 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
             const uint32_t on = so[i1];
             const uint32_t nbar0 = (uint32_t)((int)bar + dy + dz * (int)TBY) * TBW;   // only used on a contact
             nn.st = on & ~(on << 1);
-            nn.base = nbar0 + j * kLaneBlocks;
+            nn.base = (nbar0 + j * kLaneBlocks) * kNodeBytes;
             nn.first = sh[i1];
-            nn.left = last_run_node(so[i0], nn.base - kLaneBlocks, sh[i0]);
+            nn.left = last_run_node(so[i0], nn.base - kLaneBlocks * kNodeBytes, sh[i0]);
             nn.right = sh[i2];
             cs = SegContacts(A, B, Bl, Br, st, nn.st);
         };
@@ -216,17 +218,17 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
                 while (!cs.any() && q < 3) direction(++q, cs, nn);
                 if (!cs.any()) break;
                 uint32_t x;
-                const int p = cs.pop(x);
+                const int p = cs.pop_any(x);
                 const uint32_t s = hibit(st & upto(x));
-                oa = ua = s ? nb + (s >> 1) : first;
+                oa = ua = s ? node_at(nbn, s) : first;
                 ob = ub = nn.node(p);
                 busy = true;
             }
             uint32_t top;
             if (!lds_rem_step(par, ua, ub, top)) continue;
             // both original nodes now point at the node where the sets met (or lower)
-            lds_min(&par[oa], top);
-            lds_min(&par[ob], top);
+            lds_min(node_entry(par, oa), top);
+            lds_min(node_entry(par, ob), top);
             busy = false;
         }
     }
@@ -248,9 +250,9 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     uint32_t pend = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kLaneBlocks; ++k)
-        if ((st & (3u << (2 * k))) && v[k] != nb + k) pend |= 1u << k;
+        if ((st & (3u << (2 * k))) && v[k] != nbn + k * kNodeBytes) pend |= 1u << k;
     if (pend) {
-        resolve_nodes(par, nb, pend, [&](uint32_t l) {
+        resolve_nodes(par, nbn, pend, [&](uint32_t l) {
             const uint32_t q = l >> T::LOG_TBW;
             return ((bz0 + (q >> T::LOG_TBY)) * bh + by0 + (q & (TBY - 1))) * bw + bx0 + (l & (TBW - 1));
         });

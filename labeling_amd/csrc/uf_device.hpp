@@ -456,27 +456,45 @@ __device__ __forceinline__ void rows_bits32(const uint8_t *row0, const size_t (&
 // The band / bar tile kernels give each lane a 32-pixel segment (16 blocks) of
 // one row of blocks; all lanes of a row are consecutive lanes of one wave.  A
 // segment-local run of set bits is a node named by the block of its first
-// column; the forest entry of a node is par[node], parent <= node.
+// column.  Node names are byte offsets into the tile's LDS forest (4 x the
+// block's local index), so that an entry's address is its name: a parent read
+// is used as the next address with no shift.  The entry of node n is
+// node_entry(par, n), holding its parent's name, parent <= n.
 
 constexpr uint32_t kLaneBlocks = 16;          // blocks per lane (32 pixel columns)
 constexpr uint32_t kResolved = 0x80000000u;   // forest tag: entry holds the global root
+constexpr uint32_t kNodeBytes = 4;            // node name = kNodeBytes * local block index
+
+__device__ __forceinline__ uint32_t *node_entry(uint32_t *par, uint32_t n)
+{
+    return reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(par) + n);
+}
+
+// Name of the node of block s >> 1 of a segment whose first block is named base
+// (s: a column of the segment's 32).
+__device__ __forceinline__ uint32_t node_at(uint32_t base, uint32_t s)
+{
+    return base + ((s >> 1) * kNodeBytes);
+}
 
 // Node standing for the first run of segment j of a row whose first block is
 // row0 (o: the segment's run mask): the segment's own first block, or, when
 // the run continues from the segment to the left, the first block of the
 // whole run -- found with a ballot over the row's lanes (the nearest segment
 // to the left that is not all set).  All lanes of the wave must call it.
+// (row0: the name of the row's first block; returns a name.)
 __device__ __forceinline__ uint32_t run_head(uint32_t o, uint32_t j, uint32_t row0)
 {
+    constexpr uint32_t SEG = kLaneBlocks * kNodeBytes;     // name distance of consecutive segments
     const uint32_t lw = threadIdx.x & 63u, base = lw - j;   // lane of segment 0 of this row
     const uint64_t nonfull = __ballot(o != 0xFFFFFFFFu);
     const uint64_t below = nonfull & ((1ull << lw) - 1ull) & ~((1ull << base) - 1ull);
     const uint32_t k = below ? 63u - (uint32_t)__builtin_clzll(below) : lw;
     const uint32_t o_left = __shfl(o, (int)((lw + 63u) & 63u), 64);
     const uint32_t o_k = __shfl(o, (int)k, 64);
-    if (!(j > 0 && (o & 1u) && (o_left >> 31))) return row0 + j * kLaneBlocks;
-    const uint32_t sk = row0 + (k - base) * kLaneBlocks;   // segment k's first block
-    return !below ? row0 : ((o_k >> 31) ? sk + (hibit(o_k & ~(o_k << 1)) >> 1) : sk + kLaneBlocks);
+    if (!(j > 0 && (o & 1u) && (o_left >> 31))) return row0 + j * SEG;
+    const uint32_t sk = row0 + (k - base) * SEG;   // segment k's first block
+    return !below ? row0 : ((o_k >> 31) ? node_at(sk, hibit(o_k & ~(o_k << 1))) : sk + SEG);
 }
 
 // One step of uniting the sets of x and y by Rem's algorithm with splicing:
@@ -488,7 +506,7 @@ __device__ __forceinline__ uint32_t run_head(uint32_t o, uint32_t j, uint32_t ro
 // done, with `top` the node where the sets met (an ancestor of both).
 __device__ __forceinline__ bool lds_rem_step(uint32_t *par, uint32_t &x, uint32_t &y, uint32_t &top)
 {
-    uint32_t px = lds_load(&par[x]), py = lds_load(&par[y]);
+    uint32_t px = lds_load(node_entry(par, x)), py = lds_load(node_entry(par, y));
     if (px == py) {
         top = px;
         return true;
@@ -501,7 +519,7 @@ __device__ __forceinline__ bool lds_rem_step(uint32_t *par, uint32_t &x, uint32_
         px = py;
         py = tp;
     }
-    const uint32_t old = lds_min(&par[x], py);
+    const uint32_t old = lds_min(node_entry(par, x), py);
     if (old == x) {   // x was a root: now under py
         top = py;
         return true;
@@ -535,6 +553,20 @@ struct SegContacts {
         er = A & ((B >> 1) | (Br << 31)) & ~B & ~(A >> 1);
     }
     __device__ __forceinline__ uint32_t any() const { return e0 | el | er; }
+    // pop any contact: the straight ones, then the left, then the right
+    // diagonals (fewer instructions than pop; the tile kernels' unions do not
+    // depend on the order); returns the neighbour column
+    __device__ __forceinline__ int pop_any(uint32_t &xo)
+    {
+        const bool c0 = e0 != 0, cl = el != 0;
+        const uint32_t m = c0 ? e0 : (cl ? el : er);
+        const uint32_t x = __builtin_ctz(m), rest = m & (m - 1u);
+        e0 = c0 ? rest : e0;
+        el = !c0 && cl ? rest : el;
+        er = !c0 && !cl ? rest : er;
+        xo = x;
+        return (int)x + (c0 ? 0 : (cl ? -1 : 1));
+    }
     // pop the next contact: its own column x, returns the neighbour column (-1..32)
     __device__ __forceinline__ int pop(uint32_t &xo)
     {
@@ -552,7 +584,7 @@ struct SegContacts {
 // -1..32 reaches into the segments to the left / right).
 struct SegNodes {
     uint32_t st;      // run starts of segment j
-    uint32_t base;    // local index of segment j's first block
+    uint32_t base;    // name of segment j's first block
     uint32_t first;   // node standing for segment j's first run
     uint32_t left;    // node of the left segment's last run
     uint32_t right;   // node standing for the right segment's first run
@@ -561,22 +593,22 @@ struct SegNodes {
         if (p < 0) return left;
         if (p > 31) return right;
         const uint32_t s = hibit(st & upto((uint32_t)p));
-        return s ? base + (s >> 1) : first;
+        return s ? node_at(base, s) : first;
     }
 };
 
 // Node of the last run of a segment with run mask o (o != 0) whose first block
-// is base, `first` standing for its first run.
+// is named base, `first` standing for its first run.
 __device__ __forceinline__ uint32_t last_run_node(uint32_t o, uint32_t base, uint32_t first)
 {
     const uint32_t s = o ? hibit(o & ~(o << 1)) : 0u;
-    return s ? base + (s >> 1) : first;
+    return s ? node_at(base, s) : first;
 }
 
-// Chase the nodes base + k for the bits k of pend to their roots and store
-// kResolved | glob(root) in their entries (glob: local -> global block index).
-// A chase stops at an entry another lane already resolved.  No unions may run
-// concurrently.
+// Chase the nodes of blocks k (the bits of pend) of the segment whose first
+// block is named base to their roots and store kResolved | glob(root) in their
+// entries (glob: local block index -> global block index).  A chase stops at
+// an entry another lane already resolved.  No unions may run concurrently.
 template <class Glob>
 __device__ __forceinline__ void resolve_nodes(uint32_t *par, uint32_t base, uint32_t pend, const Glob &glob)
 {
@@ -585,21 +617,21 @@ __device__ __forceinline__ void resolve_nodes(uint32_t *par, uint32_t base, uint
     for (;;) {
         if (!busy) {
             if (!pend) break;
-            orig = cur = base + __builtin_ctz(pend);
+            orig = cur = base + __builtin_ctz(pend) * kNodeBytes;
             pend &= pend - 1;
             busy = true;
         }
-        const uint32_t p = lds_load(&par[cur]);
+        const uint32_t p = lds_load(node_entry(par, cur));
         uint32_t g;
         if (p & kResolved) {
             g = p;
         } else if (p == cur) {
-            g = kResolved | glob(cur);
+            g = kResolved | glob(cur / kNodeBytes);
         } else {
             cur = p;
             continue;
         }
-        lds_store(&par[orig], g);
+        lds_store(node_entry(par, orig), g);
         busy = false;
     }
 }
