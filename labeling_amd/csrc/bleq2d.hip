@@ -1037,21 +1037,77 @@ __global__ __launch_bounds__(NT_FINAL) void bleq2d_keys(Src src)
 
 // CONN4: the key is the root block index itself (a 1x2 root block holds the
 // component's first pixel, so block order is first-pixel order).
-template <class Src, bool CONN4>
+// Bit k of the result: bit k of x at 2k (even positions), of y at 2k + 1.
+__device__ __forceinline__ unsigned long long interleave32(uint32_t x, uint32_t y)
+{
+    const auto spread = [](unsigned long long v) {
+        v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+        v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+        v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+        v = (v | (v << 2)) & 0x3333333333333333ull;
+        return (v | (v << 1)) & 0x5555555555555555ull;
+    };
+    return spread(x) | (spread(y) << 1);
+}
+
+// Set the bit of every component's key.  KEY_IS_BLOCK: the key is the root
+// block's own index (4-connectivity, block-order ranks), so the 512 blocks of
+// a workgroup own bit words 8 c .. 8 c + 7 outright: each wave builds its two
+// words from ballots of its lanes' root flags and stores them whole (every
+// word is written, so the bits need no clearing first).  Otherwise the key is
+// the first-pixel key, which lies in pixel row 2 by or 2 by + 1 of the root's
+// block row by, at or right of the root's column: roots of the workgroup's
+// first block row OR their bits into two 16-word LDS windows over those rows
+// (from the workgroup's first column on), flushed with one device atomicOr per
+// non-zero word; other keys go to memory directly.  One device atomic per
+// root took 52 us at 8192^2.
+template <class Src, bool KEY_IS_BLOCK>
 __global__ __launch_bounds__(NT_FINAL) void bleq2d_mark(Src src)
 {
-    constexpr uint32_t CH = final_chunks<Src>();
+    static_assert(final_chunks<Src>() == 1 && FINAL_BLOCKS == 2 && NT_FINAL * FINAL_BLOCKS == 512,
+                  "one 512-block span per workgroup, two blocks per thread");
+    constexpr uint32_t WIN = 16;   // words per LDS window
+    __shared__ unsigned long long win[2 * WIN];
     uint32_t lw;
     const Geom2D &G = src.by_final(blockIdx.x, lw);
-    for (uint32_t ch = 0; ch < CH; ++ch) {
-        const Span sp = final_span(G, lw * CH + ch);
-        for (uint32_t k = 0; k < sp.n; ++k) {
-            const uint32_t b = sp.k0 + k;
-            const uint32_t vb = G.P[b];
-            if ((vb & 15u) && (vb >> 4) == b) {   // a root
-                const uint32_t key = CONN4 ? b : G.canon.key[b];
-                if ((key >> 6) < G.canon.nwords) atomicOr(&G.canon.bits[key >> 6], 1ull << (key & 63u));
-            }
+    const Span sp = final_span(G, lw);
+    bool root[FINAL_BLOCKS];
+#pragma unroll
+    for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+        const uint32_t b = sp.k0 + k;
+        const uint32_t vb = k < sp.n ? G.P[b] : 0u;
+        root[k] = (vb & 15u) && (vb >> 4) == b;
+    }
+    if constexpr (KEY_IS_BLOCK) {
+        const unsigned long long r0 = __ballot(root[0]), r1 = __ballot(root[1]);
+        const uint32_t lane = threadIdx.x & 63u;
+        if ((lane & 31u) == 0) {
+            const uint32_t word = (sp.k0 >> 6);   // lane 0: blocks 128 v .. + 63, lane 32: the next 64
+            const uint32_t sh = lane;             // this half's 32 lanes in the ballots
+            if (word < G.canon.nwords)
+                G.canon.bits[word] = interleave32((uint32_t)(r0 >> sh), (uint32_t)(r1 >> sh));
+        }
+    } else {
+        const uint32_t bw = G.bw, k_wg = lw * (NT_FINAL * FINAL_BLOCKS);
+        const uint32_t by0 = div_by(k_wg, G.bdiv), bx0 = k_wg - by0 * bw;
+        const uint32_t w0 = (uint32_t)(((uint64_t)(2 * by0) * bw + bx0) >> 6);
+        const uint32_t w1 = (uint32_t)(((uint64_t)(2 * by0 + 1) * bw + bx0) >> 6);
+        if (threadIdx.x < 2 * WIN) win[threadIdx.x] = 0ull;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < FINAL_BLOCKS; ++k) {
+            if (!root[k]) continue;
+            const uint32_t key = G.canon.key[sp.k0 + k], word = key >> 6;
+            const unsigned long long bit = 1ull << (key & 63u);
+            if (word - w0 < WIN) atomicOr(&win[word - w0], bit);
+            else if (word - w1 < WIN) atomicOr(&win[WIN + word - w1], bit);
+            else if (word < G.canon.nwords) atomicOr(&G.canon.bits[word], bit);
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * WIN) {
+            const unsigned long long v = win[threadIdx.x];
+            const uint32_t word = threadIdx.x < WIN ? w0 + threadIdx.x : w1 + threadIdx.x - WIN;
+            if (v && word < G.canon.nwords) atomicOr(&G.canon.bits[word], v);
         }
     }
 }
@@ -1232,10 +1288,7 @@ struct Variant {
             if (kt) kt->mark(s, "bleq2d_keys");
         }
         if (canon) {
-            if (CONN4) {   // no keys pass to clear the bits
-                hipLaunchKernelGGL((bleq2d_canon_clear<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
-                if (kt) kt->mark(s, "canon_clear");
-            }
+            // 4-connectivity keys are root blocks: the mark kernel writes every bit word
             hipLaunchKernelGGL((bleq2d_mark<Src, CONN4>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
             if (kt) kt->mark(s, "bleq2d_mark");
             hipLaunchKernelGGL((bleq2d_canon_partial<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
@@ -1244,9 +1297,7 @@ struct Variant {
             if (kt) kt->mark(s, "canon_prefix");
         }
         if (block_rank || run_bits) {
-            if (block_rank) {   // one bit per root, at the root block's own index
-                hipLaunchKernelGGL((bleq2d_canon_clear<Src>), dim3(canon_chunks), dim3(kCanonThreads), 0, s, src);
-                if (kt) kt->mark(s, "canon_clear");
+            if (block_rank) {   // one bit per root, at the root block's own index (every word written)
                 hipLaunchKernelGGL((bleq2d_mark<Src, true>), dim3(total_final), dim3(NT_FINAL), 0, s, src);
                 if (kt) kt->mark(s, "bleq2d_mark");
             } else if constexpr (Src::kSingle) {   // every bit word is written

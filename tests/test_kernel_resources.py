@@ -9,7 +9,8 @@ object metadata (no GPU needed).
 * Kernels that declare no shared arrays of their own hold at most the 64
   bytes of the count partials in LDS: the compiler may also move such an
   array into LDS instead of scratch (one slot per thread), which costs LDS
-  round trips on the store path.
+  round trips on the store path.  The 2D mark kernel declares its two
+  16-word key-bit windows (256 bytes) and nothing more.
 * The single-image tile kernels fit 64 VGPRs (8 waves per SIMD, which their
   launch bounds ask for).
 """
@@ -77,5 +78,6 @@ def test_no_register_arrays_in_lds(tmp_path):
              "bleq3d_mark", "lbeq_final", "lbeq_relabel", "lbeq2d_flatten")
     seen = {k: v.get("group_segment_fixed_size", 0) for k, v in kernels.items() if any(p in k for p in plain)}
     assert len(seen) >= 30, sorted(seen)
-    over = {k: n for k, n in seen.items() if n > 64}
+    declared = {"bleq2d_mark": 256}   # bytes of a kernel's own shared arrays
+    over = {k: n for k, n in seen.items() if n > max([64] + [v for p, v in declared.items() if p in k])}
     assert not over, over
