@@ -857,8 +857,13 @@ struct JpegDecoder {
     {
         int coef[64] = {};
         const uint16_t *q = qt[c.tq];
+        // baseline 8-bit data: DC differences of at most 11 bits, AC values of
+        // at most 10, a quantised DC within +-2047 (the level-shifted DC term
+        // lies in [-1024, 1016]); anything larger is a damaged file
         const int s = decode(dc[c.td]);
+        if (s > 11) bad("JPEG: bad DC difference size");
         c.pred += extend(bits(s), s);
+        if (c.pred < -2048 || c.pred > 2047) bad("JPEG: DC value out of range");
         coef[0] = c.pred * q[0];
         for (int k = 1; k < 64;) {
             const int rs = decode(ac[c.ta]), r = rs >> 4, sz = rs & 15;
@@ -867,6 +872,7 @@ struct JpegDecoder {
                 k += 16;
                 continue;
             }
+            if (sz > 10) bad("JPEG: bad AC value size");
             k += r;
             if (k > 63) break;
             coef[kZigzag[k]] = extend(bits(sz), sz) * q[k];
@@ -930,25 +936,28 @@ struct JpegDecoder {
             const size_t len = be16(&f[pos]), end = pos + len;
             if (end > f.size() || len < 2) bad("JPEG: truncated segment");
             const uint8_t *d = &f[pos + 2];
+            const size_t n = len - 2;   // bytes of the segment after its length
             if (m == 0xDB) {   // DQT
                 size_t i = 0;
-                while (i < len - 2) {
+                while (i < n) {
                     const int pq = d[i] >> 4, tq = d[i] & 3;
+                    if (i + 1 + (pq ? 128u : 64u) > n) bad("JPEG: bad DQT");
                     ++i;
                     for (int k = 0; k < 64; ++k) qt[tq][k] = pq ? be16(&d[i + 2 * k]) : d[i + k];
                     i += pq ? 128 : 64;
                 }
             } else if (m == 0xC4) {   // DHT
                 size_t i = 0;
-                while (i + 17 <= len - 2) {
+                while (i + 17 <= n) {
                     const int tc = d[i] >> 4, th = d[i] & 3;
                     int nv = 0;
                     for (int k = 0; k < 16; ++k) nv += d[i + 1 + k];
-                    if (i + 17 + nv > len - 2) bad("JPEG: bad DHT");
+                    if (i + 17 + nv > n) bad("JPEG: bad DHT");
                     (tc ? ac : dc)[th].build(&d[i + 1], &d[i + 17], nv);
                     i += 17 + nv;
                 }
             } else if (m == 0xC0 || m == 0xC1) {   // baseline / extended sequential Huffman
+                if (n < 6 || n < 6 + 3 * (size_t)d[5]) bad("JPEG: bad frame header");
                 if (d[0] != 8) bad("JPEG: only 8-bit samples are supported");
                 H = be16(&d[1]);
                 W = be16(&d[3]);
@@ -977,18 +986,21 @@ struct JpegDecoder {
             } else if (m >= 0xC2 && m <= 0xCF && m != 0xC4 && m != 0xC8 && m != 0xCC) {
                 bad("JPEG: progressive, lossless and arithmetic-coded files are not supported");
             } else if (m == 0xDD) {
+                if (n < 2) bad("JPEG: bad DRI");
                 restart = be16(d);
             } else if (m == 0xEE && len >= 14 && !std::memcmp(d, "Adobe", 5)) {
                 adobe_rgb = d[11] == 0;   // transform 0: RGB (or CMYK), no YCbCr
             } else if (m == 0xDA) {   // SOS
                 if (!have_frame) bad("JPEG: scan before frame");
+                if (n < 1 || n < 1 + 2 * (size_t)d[0]) bad("JPEG: bad scan header");
                 const int ns = d[0];
                 std::vector<int> ids;
                 for (int k = 0; k < ns; ++k) {
-                    const int id = d[1 + 2 * k];
+                    const int id = d[1 + 2 * k], td = d[2 + 2 * k] >> 4, ta = d[2 + 2 * k] & 15;
+                    if (td > 3 || ta > 3) bad("JPEG: bad Huffman table selector");
                     ids.push_back(id);
                     for (Comp &c : comps)
-                        if (c.id == id) { c.td = d[2 + 2 * k] >> 4; c.ta = d[2 + 2 * k] & 15; }
+                        if (c.id == id) { c.td = td; c.ta = ta; }
                 }
                 pos = end;
                 scan(ids);
