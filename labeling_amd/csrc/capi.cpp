@@ -363,8 +363,10 @@ static lbl_status launch2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &i
 {
     Options opt = ctx->opt;
     opt.status = status;
-    // lbeq with 8-connectivity runs the bleq block pipeline: bleq's tuning applies
-    const int talg = (alg == LBL_LBEQ && conn == 8) ? (int)LBL_BLEQ : (int)alg;
+    // the methods that run the bleq 2x2-block pipeline (lbeq, he-run and ocv
+    // with 8-connectivity, gr-block) take bleq's tuning
+    const bool blocks2x2 = alg == LBL_GRBLOCK || ((alg == LBL_LBEQ || alg == LBL_HERUN || alg == LBL_OCV) && conn == 8);
+    const int talg = blocks2x2 ? (int)LBL_BLEQ : (int)alg;
     auto it = ctx->tuning.find(TuneKey{talg, 0, size_class_of((uint64_t)im.h * im.w)});
     if (it != ctx->tuning.end()) {   // autotuned launch configuration for this size class
         opt.tile2d = it->second.tile;
