@@ -689,14 +689,40 @@ __global__ __launch_bounds__(NT_FINAL) void bleq3d_keys(const uint32_t *__restri
     }
 }
 
+// Set the bit of every component's key.  One thread per 16 consecutive blocks
+// (four 16-byte loads of P, one of M; P and M start 256-byte aligned), one
+// device atomicOr per root.
 __global__ __launch_bounds__(256) void bleq3d_mark(const uint32_t *__restrict__ P, const uint8_t *__restrict__ M,
                                                    uint64_t nb, Canon c)
 {
-    const uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (b >= nb) return;
-    if (M[b] && P[b] == b) {   // a root
-        const uint32_t key = c.key[b];
-        if ((key >> 6) < c.nwords) atomicOr(&c.bits[key >> 6], 1ull << (key & 63u));
+    const uint64_t b0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 16;
+    if (b0 >= nb) return;
+    uint32_t p[16];
+    uint8_t m[16];
+    if (b0 + 16 <= nb) {
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(P + b0);
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint4 x = p4[q];
+            p[4 * q] = x.x; p[4 * q + 1] = x.y; p[4 * q + 2] = x.z; p[4 * q + 3] = x.w;
+        }
+        const uint4 mm = *reinterpret_cast<const uint4 *>(M + b0);
+        const uint32_t mw[4] = {mm.x, mm.y, mm.z, mm.w};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) m[k] = (uint8_t)(mw[k >> 2] >> (8 * (k & 3)));
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) {
+            p[k] = b0 + k < nb ? P[b0 + k] : 0u;
+            m[k] = b0 + k < nb ? M[b0 + k] : (uint8_t)0;
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) {
+        if (m[k] && p[k] == b0 + k) {   // a root
+            const uint32_t key = c.key[b0 + k];
+            if ((key >> 6) < c.nwords) atomicOr(&c.bits[key >> 6], 1ull << (key & 63u));
+        }
     }
 }
 
@@ -759,7 +785,8 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
         if (kt) kt->mark(s, "bleq3d_keys");
     }
     if (canonical) {
-        hipLaunchKernelGGL(bleq3d_mark, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, P, M, nb, canon);
+        hipLaunchKernelGGL(bleq3d_mark, dim3((uint32_t)((nb + 16 * 256 - 1) / (16 * 256))), dim3(256), 0, s, P, M, nb,
+                           canon);
         if (kt) kt->mark(s, "bleq3d_mark");
         const hipError_t e = canon_scan_launch(canon, s, kt);
         if (e != hipSuccess) return e;
