@@ -377,7 +377,7 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
                           sh[r][j + 2]};   // only used when a contact exists (r > 0)
         SegContacts cs(a, sb[r][j + 1], sb[r][j], sb[r][j + 2], st, up.st);
         if (CONN4) cs.el = cs.er = 0;   // straight contacts only
-        uint32_t ua = 0, ub = 0, oa = 0, ob = 0;
+        uint32_t ua = 0, ub = 0;
         bool busy = false;
         for (;;) {
             if (!busy) {
@@ -385,16 +385,11 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
                 uint32_t x;
                 const int p = cs.pop_any(x);
                 const uint32_t s = hibit(st & upto(x));
-                oa = ua = s ? node_at(nbn, s) : first;
-                ob = ub = up.node(p);
+                ua = s ? node_at(nbn, s) : first;
+                ub = up.node(p);
                 busy = true;
             }
-            uint32_t top;
-            if (!lds_rem_step(par, ua, ub, top)) continue;
-            // both original nodes now point at the node where the sets met (or lower)
-            lds_min(node_entry(par, oa), top);
-            lds_min(node_entry(par, ob), top);
-            busy = false;
+            if (lds_rem_step(par, ua, ub)) busy = false;
         }
     }
     __syncthreads();

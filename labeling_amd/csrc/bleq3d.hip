@@ -211,7 +211,7 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         SegNodes nn{};
         uint32_t q = 0;
         direction(0, cs, nn);
-        uint32_t ua = 0, ub = 0, oa = 0, ob = 0;
+        uint32_t ua = 0, ub = 0;
         bool busy = false;
         for (;;) {
             if (!busy) {
@@ -220,16 +220,11 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
                 uint32_t x;
                 const int p = cs.pop_any(x);
                 const uint32_t s = hibit(st & upto(x));
-                oa = ua = s ? node_at(nbn, s) : first;
-                ob = ub = nn.node(p);
+                ua = s ? node_at(nbn, s) : first;
+                ub = nn.node(p);
                 busy = true;
             }
-            uint32_t top;
-            if (!lds_rem_step(par, ua, ub, top)) continue;
-            // both original nodes now point at the node where the sets met (or lower)
-            lds_min(node_entry(par, oa), top);
-            lds_min(node_entry(par, ob), top);
-            busy = false;
+            if (lds_rem_step(par, ua, ub)) busy = false;
         }
     }
     __syncthreads();

@@ -503,27 +503,26 @@ __device__ __forceinline__ uint32_t run_head(uint32_t o, uint32_t j, uint32_t ro
 // the other set) and continues from its old parent, until both have the same
 // parent or a root is linked.  It stops short of the roots when the sets are
 // already one, which most contacts of a dense image are.  Returns true when
-// done, with `top` the node where the sets met (an ancestor of both).
-__device__ __forceinline__ bool lds_rem_step(uint32_t *par, uint32_t &x, uint32_t &y, uint32_t &top)
+// done.
+//
+// A splice cuts the node's old set in two, and the steps that follow join the
+// halves again: every cut has a pending join.  No node may be moved without
+// one.  Pointing the two nodes a union started from at the node where their
+// sets met, once it is done, breaks this: the moved node can be where another
+// lane's union still stands, the only pending join of the set the node leaves;
+// that union then finds its two nodes together and stops, and the rest of
+// that set stays apart.
+__device__ __forceinline__ bool lds_rem_step(uint32_t *par, uint32_t &x, uint32_t &y)
 {
-    uint32_t px = lds_load(node_entry(par, x)), py = lds_load(node_entry(par, y));
-    if (px == py) {
-        top = px;
-        return true;
-    }
-    if (px < py) {
+    const uint32_t px = lds_load(node_entry(par, x)), py = lds_load(node_entry(par, y));
+    if (px == py) return true;
+    if (px < py) {   // the node with the larger parent moves
         const uint32_t t = x;
         x = y;
         y = t;
-        const uint32_t tp = px;
-        px = py;
-        py = tp;
     }
-    const uint32_t old = lds_min(node_entry(par, x), py);
-    if (old == x) {   // x was a root: now under py
-        top = py;
-        return true;
-    }
+    const uint32_t old = lds_min(node_entry(par, x), px < py ? px : py);
+    if (old == x) return true;   // x was a root: now under the other parent
     x = old;   // x's old parent (or whatever lowered it meanwhile) still has to join
     return false;
 }

@@ -19,8 +19,10 @@
 //
 // Variants:
 //   lds     the tile kernels' loop: lds_rem_step (Rem's algorithm with
-//           splicing, uf_device.hpp) until done, then both ORIGINAL nodes are
-//           pointed at the meeting node.
+//           splicing, uf_device.hpp) until done.
+//   lds-meet  the same, then both ORIGINAL nodes pointed at the node where the
+//           sets met (removed: it can cut a set off, see the recorded schedule
+//           below; random trials rarely reach it).
 //   rem     the global-forest unite that was tried and removed (Rem's
 //           splicing, the meeting node chased to its root, the starting nodes
 //           pointed at it when their first-read parents are larger).
@@ -33,8 +35,13 @@
 //           and a node found not to be a root continues from its fresh parent
 //           instead of being found again (which could see the same stale root
 //           forever).
+// Trials: random forests (parent <= self) of 4..12 nodes with 2..6 lanes of
+// 1..3 unions, and of 16..44 nodes with 8..24 lanes of 1..2 unions; the LDS
+// loops' unions start at any node (a run's node may have been moved by
+// another lane's union before its own contact is processed).
 // Usage: rem_interleave_check [trials]; exit code 0 = the variants the device
-// uses are exact under their memory model (lds: seq-cst; cfind: all three).
+// uses are exact under their memory model (lds: seq-cst; cfind: all three)
+// and lds-meet loses a union on the recorded schedule.
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
@@ -45,7 +52,7 @@
 
 namespace {
 
-constexpr int kMaxN = 16;
+constexpr int kMaxN = 48;
 enum Model { kSeqCst, kCoherent, kCached };
 enum Kind { kPlain, kFresh };
 
@@ -88,8 +95,8 @@ struct Shared {
     }
 };
 
-enum Variant { kLds, kRem, kFind, kCFind, kVariants };
-const char *kNames[kVariants] = {"lds", "rem", "find", "cfind"};
+enum Variant { kLds, kLdsMeet, kRem, kFind, kCFind, kVariants };
+const char *kNames[kVariants] = {"lds", "lds-meet", "rem", "find", "cfind"};
 
 // One lane running a list of unions, one shared-memory operation per step().
 struct Lane {
@@ -113,25 +120,25 @@ struct Lane {
             oa = x = jobs[next].first;
             ob = y = jobs[next].second;
             ++next;
-            pc = var == kRem ? 10 : (var == kLds ? 1 : 30);
+            pc = var == kRem ? 10 : (var == kLds || var == kLdsMeet ? 1 : 30);
         }
         switch (pc) {
         // ---- lds
         case 1: px = ld(s, x, kFresh); pc = 2; return;
         case 2:
             py = ld(s, y, kFresh);
-            if (px == py) { top = px; pc = 4; return; }
+            if (px == py) { top = px; pc = var == kLds ? 0 : 4; return; }
             if (px < py) { std::swap(x, y); std::swap(px, py); }
             pc = 3;
             return;
         case 3: {
             const uint32_t old = mn(s, x, py);
-            if (old == x) { top = py; pc = 4; return; }
+            if (old == x) { top = py; pc = var == kLds ? 0 : 4; return; }
             x = old;
             pc = 1;
             return;
         }
-        case 4: mn(s, oa, top); pc = 5; return;
+        case 4: mn(s, oa, top); pc = 5; return;   // lds-meet only
         case 5: mn(s, ob, top); pc = 0; return;
 
         // ---- rem (removed global variant)
@@ -240,7 +247,7 @@ Result run_trial(std::mt19937_64 &rng, Variant var, Model model, int n, int lane
     for (auto &l : L)
         for (int j = 0; j < jobs_per_lane; ++j) {
             uint32_t a = rng() % n, b = rng() % n;
-            if (var == kLds || var == kRem || rng() % 2) {   // unions start at nodes (tile-local roots)
+            if (var == kRem || (var != kLds && var != kLdsMeet && rng() % 2)) {   // unions start at tile-local roots
                 a = root_of(init, a);
                 b = root_of(init, b);
             }
@@ -272,13 +279,58 @@ Result run_trial(std::mt19937_64 &rng, Variant var, Model model, int n, int lane
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             if ((root_of(s.par, i) == root_of(s.par, j)) != (rfind(i) == rfind(j))) r.partition_ok = false;
-    if (var != kLds) {
+    if (var != kLds && var != kLdsMeet) {
         int links = 0, roots1 = 0;
         for (auto &l : L) links += l.links;
         for (int i = 0; i < n; ++i) roots1 += s.par[i] == (uint32_t)i;
         r.count_ok = roots0 - links == roots1;
     }
     return r;
+}
+
+// A schedule on which lds-meet loses a union (23 nodes, 12 lanes of one union
+// each; step k runs lane kSchedule[k]).  Lane 0 (13,7) stands at (5,11), the
+// only pending join of the set rooted at 1 with the set rooted at 0, when lane
+// 1 (5,17), done (its splice put 5 under 1, then 2 and 17 both had parent 0),
+// points its original node 5 at 0: lane 0 then finds 5 and 11 together and
+// stops, and the set rooted at 1 stays apart.
+const uint32_t kReplayForest[23] = {0, 1, 2, 0, 1, 2, 4, 5, 8, 6, 6, 9, 11, 13, 1, 9, 14, 13, 16, 19, 14, 21, 11};
+const uint32_t kReplayPairs[12][2] = {{13, 7}, {5, 17}, {5, 12}, {12, 5},  {14, 17}, {11, 17},
+                                      {21, 1}, {1, 20}, {19, 5}, {11, 3}, {17, 17}, {17, 22}};
+const int kSchedule[] = {11, 2, 9, 3, 0, 3, 8, 9, 6, 6, 5, 8, 0, 11, 4, 11, 3, 10, 1, 3, 3, 10, 3, 11, 8, 5, 7,
+                         11, 5, 4, 4, 4, 8, 8, 9, 1, 6, 3, 3, 6, 11, 6, 0, 11, 9, 4, 7, 0, 5, 4, 1, 5, 4, 9,
+                         11, 3, 0, 5, 7, 4, 2, 3, 3, 5, 9, 4, 3, 2, 3, 7, 4, 1, 4, 10, 4, 4, 0, 7, 1, 7, 7, 10,
+                         0, 9, 1, 1, 0, 2, 9, 0, 0, 0, 0, 0};
+
+// Runs the lanes of the recorded case on the schedule (then any lane still
+// running to its end); true when the final partition is the expected one.
+bool replay(Variant var)
+{
+    constexpr int n = 23, lanes = 12;
+    Shared s{};
+    std::mt19937_64 rng(1);
+    s.init(kReplayForest, n);
+    s.rng = &rng;
+    uint32_t ref[n];
+    std::copy(kReplayForest, kReplayForest + n, ref);
+    auto rfind = [&](uint32_t a) {
+        while (ref[a] != a) a = ref[a];
+        return a;
+    };
+    std::vector<Lane> L(lanes);
+    for (int i = 0; i < lanes; ++i) {
+        L[i].jobs.push_back({kReplayPairs[i][0], kReplayPairs[i][1]});
+        const uint32_t ra = rfind(kReplayPairs[i][0]), rb = rfind(kReplayPairs[i][1]);
+        if (ra != rb) ref[std::max(ra, rb)] = std::min(ra, rb);
+    }
+    for (const int k : kSchedule)
+        if (!L[k].done()) L[k].step(s, var);
+    for (auto &l : L)
+        while (!l.done()) l.step(s, var);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if ((root_of(s.par, i) == root_of(s.par, j)) != (rfind(i) == rfind(j))) return false;
+    return true;
 }
 
 }  // namespace
@@ -293,7 +345,10 @@ int main(int argc, char **argv)
             std::string example;
             std::mt19937_64 rng(1234 + v + 10 * m);
             for (long t = 0; t < trials; ++t) {
-                const int n = 4 + (int)(rng() % 9), lanes = 2 + (int)(rng() % 5), jobs = 1 + (int)(rng() % 3);
+                const bool large = t % 4 == 3;
+                const int n = large ? 16 + (int)(rng() % 29) : 4 + (int)(rng() % 9);
+                const int lanes = large ? 8 + (int)(rng() % 17) : 2 + (int)(rng() % 5);
+                const int jobs = large ? 1 + (int)(rng() % 2) : 1 + (int)(rng() % 3);
                 std::mt19937_64 keep = rng;
                 const Result r = run_trial(rng, (Variant)v, (Model)m, n, lanes, jobs, nullptr);
                 const bool bad[4] = {!r.partition_ok, !r.invariant_ok, !r.count_ok, !r.terminated};
@@ -313,8 +368,12 @@ int main(int argc, char **argv)
         return !fails[m][v][0] && !fails[m][v][1] && !fails[m][v][2] && !fails[m][v][3];
     };
     // the device's variants under the memory each one runs on
+    const bool meet_lost = !replay(kLdsMeet), lds_kept = replay(kLds);
+    std::printf("recorded schedule: lds-meet %s, lds %s\n", meet_lost ? "loses a union" : "exact",
+                lds_kept ? "exact" : "loses a union");
     const bool ok = !fails[kSeqCst][kLds][0] && !fails[kSeqCst][kLds][1] && !fails[kSeqCst][kLds][3] &&
-                    exact(kSeqCst, kCFind) && exact(kCoherent, kCFind) && exact(kCached, kCFind);
+                    exact(kSeqCst, kCFind) && exact(kCoherent, kCFind) && exact(kCached, kCFind) && meet_lost &&
+                    lds_kept;
     std::printf("device variants %s\n", ok ? "exact" : "FAILED");
     return ok ? 0 : 1;
 }
