@@ -46,7 +46,7 @@ struct lbl_batch {
     uint32_t total_tiles = 0, total_final = 0, total_chunks = 0;   // chunks: canonical scan (0: off)
     bool canonical = false;
     uint32_t tile = 0;
-    bool flatten = true;
+    bool flatten = false;
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -284,7 +284,7 @@ extern "C" lbl_status lbl_batch_create(lbl_ctx *ctx, lbl_alg alg, const lbl_imag
 
     const TuneKey key{(int)alg, 1, size_class_of(mean_pixels(b))};
     uint32_t tile = 0;
-    bool flatten = true;
+    bool flatten = false;
     auto it = ctx->tuning.find(key);
     if (it != ctx->tuning.end()) {
         tile = it->second.tile;

@@ -25,6 +25,13 @@ struct TileShape {
 constexpr uint32_t kTileCount = 7;
 constexpr TileShape kTiles[kTileCount] = {{128, 32}, {64, 64}, {128, 64}, {256, 32}, {256, 64}, {512, 32}, {128, 128}};
 
+// Tile shape of a single image with no tuning-table entry for its size class,
+// from a sweep of every shape on MI355X (square images of 2^18 .. 2^27 pixels,
+// Bernoulli p = 0.1 .. 0.9 and blobs): 256x32 blocks from 2^23 pixels up (the
+// best there, or within 4%), 128x32 below (within 0.5% of the best).  The
+// flatten pass is off by default: it cost 3 to 40 us at every size.
+constexpr uint32_t default_tile2d(uint64_t pixels) { return pixels >= (1ull << 23) ? 3u : 0u; }
+
 // Final-kernel (and key / mark kernel) workgroups: 256 threads x 2 blocks each
 // per chunk, over the image's blocks in raster order; a single image's
 // workgroups take one chunk each, a batch's kBatchFinalChunks consecutive

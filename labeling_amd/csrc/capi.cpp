@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 
+#include "batch_plan.hpp"
 #include "lbl_internal.hpp"
 
 using namespace lbl;
@@ -371,6 +372,8 @@ static lbl_status launch2d(lbl_ctx *ctx, lbl_alg alg, int conn, const Image2D &i
     if (it != ctx->tuning.end()) {   // autotuned launch configuration for this size class
         opt.tile2d = it->second.tile;
         opt.flatten = it->second.flatten;
+    } else {
+        opt.tile2d = plan::default_tile2d((uint64_t)im.h * im.w);
     }
     hipError_t e = hipSuccess;
     switch (alg) {

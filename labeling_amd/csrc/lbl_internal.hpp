@@ -35,8 +35,8 @@ struct Options {
     lbl_label_mode mode = LBL_LABELS_ROOT;
     uint32_t iter_cap = 0;       // lbeq tile sweeps cap, 0 = none
     uint32_t *status = nullptr;  // device word; kernels OR in error bits (bit0 = iteration cap hit)
-    uint32_t tile2d = 0;         // bleq 2D tile shape id (kBleq2dTiles)
-    bool flatten = true;         // run the flatten pass (else the final pass chases full paths)
+    uint32_t tile2d = 0;         // bleq 2D tile shape id (kBleq2dTiles; per call: tuning table, else plan::default_tile2d)
+    bool flatten = false;        // run the flatten pass (else the final pass chases full paths)
 };
 
 // Autotuned launch configuration for a workload class.

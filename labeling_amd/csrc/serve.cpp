@@ -22,7 +22,7 @@ namespace {
 struct HipReq : serve::Req {
     Options opt;            // label mode, iteration cap, single-image tile shape and flatten
     uint32_t batch_tile = 0;   // launch configuration when fused (batch tuning of its size class)
-    bool batch_flatten = true;
+    bool batch_flatten = false;
     bool is3d() const { return r.d != 0; }
     bool canonical() const { return mode == LBL_LABELS_CANONICAL; }
     uint64_t elems() const { return (uint64_t)(r.d ? r.d : 1) * r.h * r.w; }
@@ -300,6 +300,8 @@ extern "C" lbl_status lbl_server_submit(lbl_server *srv, const lbl_request *req,
             if (it != ctx->tuning.end()) {
                 r->opt.tile2d = it->second.tile;
                 r->opt.flatten = it->second.flatten;
+            } else {
+                r->opt.tile2d = plan::default_tile2d((uint64_t)q.h * q.w);
             }
             auto bt = ctx->tuning.find(TuneKey{(int)LBL_BLEQ, 1, cls});
             if (bt != ctx->tuning.end()) {
