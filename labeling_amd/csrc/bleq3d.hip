@@ -122,13 +122,30 @@ __device__ __forceinline__ uint32_t block_key3(uint32_t h, uint32_t w, uint32_t 
     return voxel_key ? zy * w + 2 * bx + dx : zy * bw + bx;
 }
 
+// x-boundary planes: for each boundary k = 1 .. tiles_x - 1 between tile
+// columns k - 1 and k (block column x = k * TX), the (tile-local root, mask) of
+// every block of the column just right of it (west plane: tile column k's
+// first) and just left of it (east plane: tile column k - 1's last), entry
+// z * bh + y.  The tile kernel writes them with its block words; the merge's
+// x-face lanes read them instead of those columns of P and M, whose blocks
+// each lie in a cache line of their own.
+__device__ __forceinline__ uint2 *xplane_west(uint2 *xf, uint32_t bd, uint32_t bh, uint32_t k)
+{
+    return xf + (size_t)(k - 1) * bd * bh;
+}
+
+__device__ __forceinline__ uint2 *xplane_east(uint2 *xf, uint32_t tiles_x, uint32_t bd, uint32_t bh, uint32_t k)
+{
+    return xf + (size_t)(tiles_x - 1 + k - 1) * bd * bh;
+}
+
 // VEC: 16-byte vector loads of the rows; FULL: w % 32 == 0, so every lane's
 // segments are whole and the kernel holds no code for row ends.
 template <uint32_t TBW, uint32_t TBY, uint32_t TBZ, bool VEC, bool FULL>
 __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
     const uint8_t *__restrict__ vox, uint32_t d, uint32_t h, uint32_t w, uint32_t bd, uint32_t bh, uint32_t bw,
-    uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P, uint8_t *__restrict__ M, uint32_t *count,
-    uint32_t *slots, uint32_t *ckey, bool voxel_key)
+    uint32_t tiles_x, uint32_t tiles_y, uint32_t *__restrict__ P, uint8_t *__restrict__ M, uint2 *__restrict__ xf,
+    uint32_t *count, uint32_t *slots, uint32_t *ckey, bool voxel_key)
 {
     using T = Tile3<TBW, TBY, TBZ>;
     constexpr uint32_t LPB = T::LPB;
@@ -297,6 +314,12 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
                 mdst[k] = (uint8_t)(mk[k >> 2] >> (8 * (k & 3)));
             }
         }
+        // the tile's first and last block columns, also into the x-boundary
+        // planes for the merge (xplane_west / xplane_east)
+        if (j == 0 && tx > 0) xplane_west(xf, bd, bh, tx)[(size_t)bz * bh + by] = make_uint2(out[0], mask_of(0));
+        if (j == LPB - 1 && tx + 1 < tiles_x)
+            xplane_east(xf, tiles_x, bd, bh, tx + 1)[(size_t)bz * bh + by] =
+                make_uint2(out[kLaneBlocks - 1], mask_of(kLaneBlocks - 1));
     }
     if (count) count_add_wave(slots, roots);
 }
@@ -365,8 +388,8 @@ __device__ __forceinline__ void bar_rows(const uint8_t *m, uint32_t bxs, uint32_
     }
 }
 
-__global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const uint8_t *__restrict__ M, Grid3 g,
-                                                              uint32_t *count, uint32_t *slots)
+__global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const uint8_t *__restrict__ M, uint2 *xf,
+                                                              Grid3 g, uint32_t *count, uint32_t *slots)
 {
     constexpr uint32_t LPB = TX / kLaneBlocks;
     __shared__ LdsPairSet<PAIR_CAP> set;   // distinct pairs of roots
@@ -431,16 +454,18 @@ __global__ __launch_bounds__(MERGE_THREADS) void bleq3d_merge(uint32_t *P, const
             const uint32_t bx0 = (tile % g.tiles_x) * TX, by = ((tile / g.tiles_x) % g.tiles_y) * TY + ly;
             const uint32_t bz = (tile / (g.tiles_x * g.tiles_y)) * TZ + lz;
             if (bx0 > 0 && bx0 < g.bw && by < g.bh && bz < g.bd) {
-                const size_t b0 = ((size_t)bz * g.bh + by) * g.bw + bx0;
-                const uint32_t m = M[b0];
+                const uint32_t kx = tile % g.tiles_x;   // the boundary on this tile's left
+                const uint2 *W = xplane_west(xf, g.bd, g.bh, kx);
+                const uint2 *E = xplane_east(xf, g.tiles_x, g.bd, g.bh, kx);
+                const uint2 w0 = W[(size_t)bz * g.bh + by];
+                const uint32_t m = w0.y;
                 if (m & 0x55u) {
-                    const uint32_t rb0 = P[b0];
                     for (int e9 = 0; e9 < 9; ++e9) {
                         const int dy = e9 % 3 - 1, dz = e9 / 3 - 1;
                         const uint32_t ny = by + dy, nz = bz + dz;
                         if (ny >= g.bh || nz >= g.bd) continue;
-                        const size_t n = ((size_t)nz * g.bh + ny) * g.bw + bx0 - 1;
-                        if (adjacent(m, M[n], -1, dy, dz)) push(rb0, P[n]);
+                        const uint2 en = E[(size_t)nz * g.bh + ny];
+                        if (adjacent(m, en.y, -1, dy, dz)) push(w0.x, en.x);
                     }
                 }
             }
@@ -723,10 +748,17 @@ __global__ __launch_bounds__(256) void bleq3d_mark(const uint32_t *__restrict__ 
 
 }  // namespace
 
+// Bytes of the x-boundary planes (west and east for each of tiles_x - 1 boundaries).
+static uint64_t xplane_bytes(uint64_t bd, uint64_t bh, uint64_t bw)
+{
+    const uint64_t tiles_x = (bw + TX - 1) / TX;
+    return 2 * (tiles_x - 1) * bd * bh * sizeof(uint2);
+}
+
 size_t bleq3d_workspace(uint32_t d, uint32_t h, uint32_t w, bool canonical)
 {
-    const uint64_t nb = ((d + 1ull) / 2) * ((h + 1ull) / 2) * ((w + 1ull) / 2);
-    return align256(kCountBytes) + align256(nb * sizeof(uint32_t)) + align256(nb) +
+    const uint64_t bd = (d + 1ull) / 2, bh = (h + 1ull) / 2, bw = (w + 1ull) / 2, nb = bd * bh * bw;
+    return align256(kCountBytes) + align256(nb * sizeof(uint32_t)) + align256(nb) + align256(xplane_bytes(bd, bh, bw)) +
            (canonical ? canon_workspace((uint64_t)d * h * ((w + 1ull) / 2), nb) : 0);
 }
 
@@ -746,8 +778,10 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
     uint32_t *slots = static_cast<uint32_t *>(ws);   // count slots first: zero between runs
     uint32_t *P = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(ws) + align256(kCountBytes));
     uint8_t *M = reinterpret_cast<uint8_t *>(P) + align256(nb * sizeof(uint32_t));
+    uint2 *xf = reinterpret_cast<uint2 *>(M + align256(nb));
+    uint8_t *after_planes = reinterpret_cast<uint8_t *>(xf) + align256(xplane_bytes(bd, bh, bw));
     const bool canonical = opt.mode == LBL_LABELS_CANONICAL, keys = canonical || first;
-    Canon canon = keys ? canon_layout(M + align256(nb), (uint64_t)im.d * im.h * bw, nb) : Canon{};
+    Canon canon = keys ? canon_layout(after_planes, (uint64_t)im.d * im.h * bw, nb) : Canon{};
     if (!canonical) canon.nwords = 0;   // keys only
     const bool ref = opt.mode == LBL_LABELS_REFERENCE || opt.mode == LBL_LABELS_REFERENCE_UNCROPPED;
     const bool uncropped = opt.mode == LBL_LABELS_REFERENCE_UNCROPPED;
@@ -762,15 +796,15 @@ static hipError_t run_blocks3d(const Image3D &im, const Options &opt, void *ws, 
     constexpr uint32_t NT_TILE = Tile3<TX, TY, TZ>::THREADS;
     if (vec_in && im.w % 32 == 0)
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true, true>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d,
-                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
+                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, xf, im.count, slots, canon.key, !canonical);
     else if (vec_in)
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, true, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d,
-                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
+                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, xf, im.count, slots, canon.key, !canonical);
     else
         hipLaunchKernelGGL((bleq3d_tile<TX, TY, TZ, false, false>), dim3(g.ntiles), dim3(NT_TILE), 0, s, im.vox, im.d,
-                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, im.count, slots, canon.key, !canonical);
+                           im.h, im.w, bd, bh, bw, g.tiles_x, g.tiles_y, P, M, xf, im.count, slots, canon.key, !canonical);
     if (kt) kt->mark(s, "bleq3d_tile");
-    hipLaunchKernelGGL(bleq3d_merge, dim3(merge_grid), dim3(MERGE_THREADS), 0, s, P, M, g, im.count, slots);
+    hipLaunchKernelGGL(bleq3d_merge, dim3(merge_grid), dim3(MERGE_THREADS), 0, s, P, M, xf, g, im.count, slots);
     if (kt) kt->mark(s, "bleq3d_merge");
     const dim3 fgrid((uint32_t)((uint64_t)xgroups * bh * bd));
     if (keys) {
