@@ -377,7 +377,9 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
                           sh[r][j + 2]};   // only used when a contact exists (r > 0)
         SegContacts cs(a, sb[r][j + 1], sb[r][j], sb[r][j + 2], st, up.st);
         if (CONN4) cs.el = cs.er = 0;   // straight contacts only
-        uint32_t ua = 0, ub = 0;
+        // a contact sharing a node with the lane's previous union starts that
+        // side from the node where the previous union met
+        uint32_t ua = 0, ub = 0, la = ~0u, lb = ~0u, ltop = 0;
         bool busy = false;
         for (;;) {
             if (!busy) {
@@ -385,11 +387,14 @@ __global__ __launch_bounds__((TBW / LANE_BLOCKS) * TBH, (Src::kSingle && TBW >= 
                 uint32_t x;
                 const int p = cs.pop_any(x);
                 const uint32_t s = hibit(st & upto(x));
-                ua = s ? node_at(nbn, s) : first;
-                ub = up.node(p);
+                const uint32_t na = s ? node_at(nbn, s) : first, nu = up.node(p);
+                ua = na == la ? ltop : na;
+                ub = nu == lb ? ltop : nu;
+                la = na;
+                lb = nu;
                 busy = true;
             }
-            if (lds_rem_step(par, ua, ub)) busy = false;
+            if (lds_rem_step(par, ua, ub, ltop)) busy = false;
         }
     }
     __syncthreads();

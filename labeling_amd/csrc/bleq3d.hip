@@ -228,7 +228,10 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
         SegNodes nn{};
         uint32_t q = 0;
         direction(0, cs, nn);
-        uint32_t ua = 0, ub = 0;
+        // (starting a contact that shares a node with the lane's previous union
+        // from where that union met, as the 2D tile kernel does, made this
+        // kernel slower: 134 -> 141 us at 512^3)
+        uint32_t ua = 0, ub = 0, top = 0;
         bool busy = false;
         for (;;) {
             if (!busy) {
@@ -241,7 +244,7 @@ __global__ __launch_bounds__((TBW / kLaneBlocks) * TBY * TBZ) void bleq3d_tile(
                 ub = nn.node(p);
                 busy = true;
             }
-            if (lds_rem_step(par, ua, ub)) busy = false;
+            if (lds_rem_step(par, ua, ub, top)) busy = false;
         }
     }
     __syncthreads();

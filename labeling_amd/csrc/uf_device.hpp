@@ -503,7 +503,8 @@ __device__ __forceinline__ uint32_t run_head(uint32_t o, uint32_t j, uint32_t ro
 // the other set) and continues from its old parent, until both have the same
 // parent or a root is linked.  It stops short of the roots when the sets are
 // already one, which most contacts of a dense image are.  Returns true when
-// done.
+// done, with `top` the node where the sets met: a member of the joined set,
+// from which a lane's next union on the same node may start (no node moves).
 //
 // A splice cuts the node's old set in two, and the steps that follow join the
 // halves again: every cut has a pending join.  No node may be moved without
@@ -512,16 +513,17 @@ __device__ __forceinline__ uint32_t run_head(uint32_t o, uint32_t j, uint32_t ro
 // lane's union still stands, the only pending join of the set the node leaves;
 // that union then finds its two nodes together and stops, and the rest of
 // that set stays apart.
-__device__ __forceinline__ bool lds_rem_step(uint32_t *par, uint32_t &x, uint32_t &y)
+__device__ __forceinline__ bool lds_rem_step(uint32_t *par, uint32_t &x, uint32_t &y, uint32_t &top)
 {
     const uint32_t px = lds_load(node_entry(par, x)), py = lds_load(node_entry(par, y));
+    top = px < py ? px : py;
     if (px == py) return true;
     if (px < py) {   // the node with the larger parent moves
         const uint32_t t = x;
         x = y;
         y = t;
     }
-    const uint32_t old = lds_min(node_entry(par, x), px < py ? px : py);
+    const uint32_t old = lds_min(node_entry(par, x), top);
     if (old == x) return true;   // x was a root: now under the other parent
     x = old;   // x's old parent (or whatever lowered it meanwhile) still has to join
     return false;

@@ -20,6 +20,8 @@
 // Variants:
 //   lds     the tile kernels' loop: lds_rem_step (Rem's algorithm with
 //           splicing, uf_device.hpp) until done.
+//   lds-top the same, but a union whose node is the previous union's same
+//           node starts there from the node where the previous union met.
 //   lds-meet  the same, then both ORIGINAL nodes pointed at the node where the
 //           sets met (removed: it can cut a set off, see the recorded schedule
 //           below; random trials rarely reach it).
@@ -38,7 +40,8 @@
 // Trials: random forests (parent <= self) of 4..12 nodes with 2..6 lanes of
 // 1..3 unions, and of 16..44 nodes with 8..24 lanes of 1..2 unions; the LDS
 // loops' unions start at any node (a run's node may have been moved by
-// another lane's union before its own contact is processed).
+// another lane's union before its own contact is processed), and half of a
+// lane's later unions share a node with its union before.
 // Usage: rem_interleave_check [trials]; exit code 0 = the variants the device
 // uses are exact under their memory model (lds: seq-cst; cfind: all three)
 // and lds-meet loses a union on the recorded schedule.
@@ -95,8 +98,8 @@ struct Shared {
     }
 };
 
-enum Variant { kLds, kLdsMeet, kRem, kFind, kCFind, kVariants };
-const char *kNames[kVariants] = {"lds", "lds-meet", "rem", "find", "cfind"};
+enum Variant { kLds, kLdsTop, kLdsMeet, kRem, kFind, kCFind, kVariants };
+const char *kNames[kVariants] = {"lds", "lds-top", "lds-meet", "rem", "find", "cfind"};
 
 // One lane running a list of unions, one shared-memory operation per step().
 struct Lane {
@@ -104,6 +107,7 @@ struct Lane {
     size_t next = 0;
     int pc = 0;   // 0 = start the next job
     uint32_t oa = 0, ob = 0, x = 0, y = 0, px = 0, py = 0, top = 0, v0x = 0, v0y = 0, chase = 0;
+    uint32_t last_oa = ~0u, last_ob = ~0u, last_top = 0;   // lds-top: the previous union's nodes and meeting node
     // find sub-machine: chases from fstart, result in froot; `ret` resumes the caller
     uint32_t fstart = 0, fcur = 0, fv0 = 0, fv = 0, froot = 0, fa = 0, fb = 0;
     int ret = 0;
@@ -111,6 +115,16 @@ struct Lane {
     size_t seen[kMaxN] = {};
     bool done() const { return next >= jobs.size() && pc == 0; }
     uint32_t ld(Shared &s, uint32_t i, Kind k) { return s.load(i, seen[i], k); }
+    // end of an LDS union: lds-meet goes on to its last steps; lds-top keeps
+    // the meeting node for the next union
+    int done_lds(Variant var)
+    {
+        if (var == kLdsMeet) return 4;
+        last_oa = oa;
+        last_ob = ob;
+        last_top = top;
+        return 0;
+    }
     uint32_t mn(Shared &s, uint32_t i, uint32_t v) { return s.amin(i, v, seen[i]); }
 
     void step(Shared &s, Variant var)
@@ -120,20 +134,24 @@ struct Lane {
             oa = x = jobs[next].first;
             ob = y = jobs[next].second;
             ++next;
-            pc = var == kRem ? 10 : (var == kLds || var == kLdsMeet ? 1 : 30);
+            if (var == kLdsTop) {   // a node shared with the previous union: start from where it met
+                if (oa == last_oa) x = last_top;
+                if (ob == last_ob) y = last_top;
+            }
+            pc = var == kRem ? 10 : (var == kLds || var == kLdsTop || var == kLdsMeet ? 1 : 30);
         }
         switch (pc) {
         // ---- lds
         case 1: px = ld(s, x, kFresh); pc = 2; return;
         case 2:
             py = ld(s, y, kFresh);
-            if (px == py) { top = px; pc = var == kLds ? 0 : 4; return; }
+            if (px == py) { top = px; pc = done_lds(var); return; }
             if (px < py) { std::swap(x, y); std::swap(px, py); }
             pc = 3;
             return;
         case 3: {
             const uint32_t old = mn(s, x, py);
-            if (old == x) { top = py; pc = var == kLds ? 0 : 4; return; }
+            if (old == x) { top = py; pc = done_lds(var); return; }
             x = old;
             pc = 1;
             return;
@@ -247,9 +265,15 @@ Result run_trial(std::mt19937_64 &rng, Variant var, Model model, int n, int lane
     for (auto &l : L)
         for (int j = 0; j < jobs_per_lane; ++j) {
             uint32_t a = rng() % n, b = rng() % n;
-            if (var == kRem || (var != kLds && var != kLdsMeet && rng() % 2)) {   // unions start at tile-local roots
+            if (var == kRem || (var != kLds && var != kLdsTop && var != kLdsMeet && rng() % 2)) {   // unions start at tile-local roots
                 a = root_of(init, a);
                 b = root_of(init, b);
+            }
+            // a lane's contacts often share a node with the one before (a run
+            // touching several runs of the band above, or the reverse)
+            if (j > 0 && (var == kLds || var == kLdsTop || var == kLdsMeet) && rng() % 2) {
+                if (rng() % 2) a = l.jobs.back().first;
+                else b = l.jobs.back().second;
             }
             l.jobs.push_back({a, b});
             const uint32_t ra = rfind(a), rb = rfind(b);
@@ -279,7 +303,7 @@ Result run_trial(std::mt19937_64 &rng, Variant var, Model model, int n, int lane
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             if ((root_of(s.par, i) == root_of(s.par, j)) != (rfind(i) == rfind(j))) r.partition_ok = false;
-    if (var != kLds && var != kLdsMeet) {
+    if (var != kLds && var != kLdsTop && var != kLdsMeet) {
         int links = 0, roots1 = 0;
         for (auto &l : L) links += l.links;
         for (int i = 0; i < n; ++i) roots1 += s.par[i] == (uint32_t)i;
@@ -372,6 +396,7 @@ int main(int argc, char **argv)
     std::printf("recorded schedule: lds-meet %s, lds %s\n", meet_lost ? "loses a union" : "exact",
                 lds_kept ? "exact" : "loses a union");
     const bool ok = !fails[kSeqCst][kLds][0] && !fails[kSeqCst][kLds][1] && !fails[kSeqCst][kLds][3] &&
+                    !fails[kSeqCst][kLdsTop][0] && !fails[kSeqCst][kLdsTop][1] && !fails[kSeqCst][kLdsTop][3] &&
                     exact(kSeqCst, kCFind) && exact(kCoherent, kCFind) && exact(kCached, kCFind) && meet_lost &&
                     lds_kept;
     std::printf("device variants %s\n", ok ? "exact" : "FAILED");

@@ -4,9 +4,11 @@ merges' unite (finds by cacheable loads, root check by a device-coherent load,
 root link by atomicMin) keep every union under random interleavings of single
 loads and atomicMins: the LDS loop with sequentially consistent memory, the
 global unite also with loads that return any earlier value of their location.
-The LDS loop's former last step (both original nodes pointed at the node where
-the sets met) loses a union on a recorded schedule of 12 lanes, which the
-check replays: it must fail there while the loop without that step keeps it."""
+The 2D loop's start of a union from the node where the lane's previous union
+met (lds-top) keeps every union too.  The LDS loop's former last step (both
+original nodes pointed at the node where the sets met) loses a union on a
+recorded schedule of 12 lanes, which the check replays: it must fail there
+while the loop without that step keeps it."""
 import os
 import subprocess
 
